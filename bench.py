@@ -1,0 +1,194 @@
+"""Benchmark: time to 1e-10 relative residual at 512^3, mixed precision vs FP64.
+
+Workload: -lap(u) = f on the unit cube, 512^3 cells, homogeneous Dirichlet,
+7-point stencil, b = -f with f = U[-1,1] (SplitMix64, seed 20210501) +
+3 pi^2 sin(pi x) sin(pi y) sin(pi z); V(2,2) red/black Gauss-Seidel
+cycles (omega = 1), 8-child volume-weighted restriction, trilinear
+prolongation, 7 levels down to 4^3, coarsest solved to the tolerance.
+
+A step is one complete solve from x = 0 to rho < 1e-10 with the inputs
+already resident in HBM.  value = mean wall time of a mixed-precision step
+(FP32 V-cycles inside FP64 iterative refinement); the FP64 solve is timed
+the same way and reported beside it.  The roofline object is for the
+fine-level smoother kernel, timed with HIP events on the solve stream.
+cpu_baseline times the oracle (the C restatement of the reference loops,
+OpenMP) on two V-cycles of the same 512^3 workload and extrapolates to the
+FP64 cycle count.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--n", type=int, default=512)
+    p.add_argument("--lz", type=float, default=1.0, help="domain length in z (2.0: hx:hy:hz = 1:1:2)")
+    p.add_argument("--inner", type=int, default=0, help="FP32 V-cycles per refinement step (0: default)")
+    p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    p.add_argument("--smoother-reps", type=int, default=20)
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, lz, fp64_cycles):
+    """Oracle (C restatement, OpenMP) on 2 V-cycles of the same workload."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    g = O.Grid(n, n, n, l=(1.0, 1.0, lz))
+    A = g.matrix()
+    b = O.uniform_field(n, n, n, 20210501)
+    s = np.sin(np.pi * g.xg)[None, None, 1:-1] * np.sin(np.pi * g.yg)[None, 1:-1, None] * \
+        np.sin(np.pi * g.zg)[1:-1, None, None]
+    b[1:-1, 1:-1, 1:-1] = -(b[1:-1, 1:-1, 1:-1] + 3 * np.pi ** 2 * s)
+    levels = int(round(math.log2(n // 4)))
+    ncyc = 2
+    mg = O.Multigrid(g, levels, ncyc)
+    x = O.padded(n, n, n)
+    r = O.padded(n, n, n)
+    t0 = time.perf_counter()
+    nc, hist, cit, res0 = mg.solve_vcycle(x, r, A, b, 2, 1e-30, 1.0)
+    dt = time.perf_counter() - t0
+    per_cycle = dt / nc
+    return {"value": per_cycle * fp64_cycles, "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of the reference loops, AoS coefficients, OpenMP {threads} threads) "
+                      f"ran {nc} V(2,2) cycles at {n}^3 in {dt:.2f} s ({per_cycle:.2f} s/cycle); "
+                      f"value = s/cycle x {fp64_cycles} cycles the FP64 solve needs"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
+                                   "and not implemented", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from peps_mg_amd import _lib, geometry
+    from peps_mg_amd.multigrid import Multigrid
+
+    n, lz = a.n, a.lz
+    levels = int(round(math.log2(n // 4)))
+    gdm = geometry.geometry_domain_create(n, n, n, 0.0, 0.0, 0.0, 1.0, 1.0, lz)
+    sdm = geometry.geometry_subdomain_create(gdm)
+    dev = torch.device("cuda:0")
+    d_b = torch.zeros((n + 2,) * 3, dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib().pepsmg_fill_rhs_device(n, n, n, C.c_void_p(d_b.data_ptr()), 20210501, -1.0,
+                                                 -3 * math.pi ** 2, _lib.dptr(sdm.xg), _lib.dptr(sdm.yg),
+                                                 _lib.dptr(sdm.zg), None))
+    d_x = torch.zeros_like(d_b)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def make(precision):
+        kw = dict(precision=precision)
+        if precision == 1 and a.inner:
+            kw["inner_cycles"] = a.inner
+        return Multigrid(sdm, levels, 200, **kw)
+
+    def run(mg, steps, warmup, label):
+        res = None
+        for _ in range(warmup):
+            d_x.zero_()
+            res = mg.solve_vcycle_device(d_x.data_ptr(), 0, d_b.data_ptr(), 2, 1e-10, 1.0, stream)
+        torch.cuda.synchronize()
+        times = []
+        t_all = time.perf_counter()
+        for q in range(steps):
+            t0 = time.perf_counter()
+            d_x.zero_()
+            res = mg.solve_vcycle_device(d_x.data_ptr(), 0, d_b.data_ptr(), 2, 1e-10, 1.0, stream)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            log(f"[bench] {label} step {q}: {times[-1]*1e3:.2f} ms, cycles={res.cycles}, rho={res.rho:.3e}")
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t_all
+        return total / steps, res
+
+    log(f"[bench] {n}^3 lz={lz} levels={levels}")
+    mg64 = make(0)
+    t64, r64 = run(mg64, a.steps, a.warmup, "fp64")
+    x64 = d_x.clone()
+    mgmx = make(1)
+    tmx, rmx = run(mgmx, a.steps, a.warmup, "mixed")
+    err_max = float((d_x - x64).abs().max() / x64.abs().max())
+    err_l2 = float(torch.linalg.vector_norm(d_x - x64) / torch.linalg.vector_norm(x64))
+
+    # fine-level smoother kernel, timed with events on the solve stream
+    avg = C.c_double(0)
+    nbytes = C.c_double(0)
+    kname = C.c_char_p()
+    _lib.check(_lib.lib().pepsmg_bench_smoother(mg64.h, 0, a.smoother_reps, 1.0, C.byref(avg), C.byref(nbytes),
+                                                C.byref(kname)))
+    achieved = nbytes.value / (avg.value * 1e-3) / 1e9
+    avg32 = C.c_double(0)
+    nbytes32 = C.c_double(0)
+    kname32 = C.c_char_p()
+    _lib.check(_lib.lib().pepsmg_bench_smoother(mgmx.h, 1, a.smoother_reps, 1.0, C.byref(avg32),
+                                                C.byref(nbytes32), C.byref(kname32)))
+    achieved32 = nbytes32.value / (avg32.value * 1e-3) / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] cpu_baseline: oracle, 2 V-cycles")
+        del mg64, mgmx
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(n, lz, r64.cycles)
+
+    unknowns = n ** 3
+    out = {
+        "metric": "time to 1e-10 rel. residual at 512^3 (mixed vs FP64 GPU); smoother HBM GB/s",
+        "value": round(tmx, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(tmx * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"Poisson 7-point, {n}^3 cells, unit cube x (1,1,{lz:g}), homogeneous Dirichlet, "
+                               f"V(2,2) RBGS omega=1, {levels} coarse levels to 4^3, tol 1e-10, x0=0",
+                   "grid": [n, n, n], "levels": levels, "precision": "fp32 V-cycles in fp64 refinement"},
+        "fp64_time_s": round(t64, 6),
+        "fp64_cycles": r64.cycles,
+        "mixed_time_s": round(tmx, 6),
+        "mixed_cycles": rmx.cycles,
+        "mixed_refine_steps": rmx.refine_steps,
+        "speedup_mixed_vs_fp64": round(t64 / tmx, 4),
+        "fp64_vcycles_per_s": round(r64.cycles / t64, 3),
+        "fp64_unknowns_per_s": round(unknowns * r64.cycles / t64, 1),
+        "mixed_unknowns_per_s": round(unknowns * rmx.cycles / tmx, 1),
+        "rho_fp64": r64.rho,
+        "rho_mixed": rmx.rho,
+        "mixed_vs_fp64_error": {"max_rel": err_max, "l2_rel": err_l2},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": kname.value.decode(), "avg_ms": avg.value, "bytes_per_launch": nbytes.value},
+        "roofline_fp32_smoother": {"achieved": round(achieved32, 1), "frac": round(achieved32 / HBM_PEAK_GBS, 4),
+                                   "kernel": kname32.value.decode(), "avg_ms": avg32.value,
+                                   "bytes_per_launch": nbytes32.value},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+/*
+ * pepsmg.h -- C-ABI of libpepsmg.so, the MI355X implementation of PEPS_MG's
+ * geometric-multigrid path (3-D cell-centred 7-point Poisson operator,
+ * homogeneous Dirichlet boundaries, 1 process owning the whole grid).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status (PEPSMG_OK = 0, negative on error; the message is
+ * available from pepsmg_last_error()).  Host-pointer entry points take the
+ * reference's padded Fortran arrays a(0:nx+1,0:ny+1,0:nz+1) (x fastest,
+ * contiguous, as allocated by geometry_subdomain_create,
+ * src/geometry.f90:415-421) and copy them to and from the GPU; the *_device
+ * variants take device pointers in the same padded layout and a HIP stream.
+ *
+ * Ghost cells of solution arrays must be zero (homogeneous Dirichlet, the
+ * only boundary condition the reference's driver sets up); the library
+ * checks this and returns PEPSMG_ERR_GHOST otherwise.
+ *
+ * Matrices: the reference passes its AoS coefficient array
+ * coeff(0:6,nx,ny,nz) (src/matrix.f90:27-30).  The GPU path is matrix-free:
+ * it recomputes the seven coefficients from the grid metrics with the
+ * reference formula (src/matrix.f90:78-88).  When a coeff pointer is given
+ * it is compared with that recomputation and PEPSMG_ERR_MATRIX is returned
+ * if any coefficient differs.  NULL skips the comparison.
+ */
+#ifndef PEPSMG_H
+#define PEPSMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PEPSMG_OK = 0,
+  PEPSMG_ERR_ARG = -1,        /* invalid argument or size                     */
+  PEPSMG_ERR_LEVELS = -2,     /* hierarchy rules of multigrid_create violated */
+  PEPSMG_ERR_GHOST = -3,      /* non-zero ghost cell in a solution array      */
+  PEPSMG_ERR_MATRIX = -4,     /* coeff differs from the metric rediscretisation */
+  PEPSMG_ERR_HIP = -5,        /* HIP runtime error                            */
+  PEPSMG_ERR_NOMEM = -6,      /* device allocation failed                     */
+  PEPSMG_ERR_NOTCONV = -7     /* solver stopped at its cap above tolerance (solver entry points only) */
+};
+
+/* One subdomain's metrics (type subdomain, src/geometry.f90:59-118): cell
+ * counts and the 1-D arrays dxm/dxg (cell size, centre spacing) of each
+ * axis, each of length n+2 including the two ghost entries. */
+typedef struct pepsmg_grid {
+  int32_t nx, ny, nz;
+  const double *dxm, *dxg;
+  const double *dym, *dyg;
+  const double *dzm, *dzg;
+} pepsmg_grid;
+
+/* Solver options with no counterpart in the reference (its V-cycle always
+ * uses RBGS in FP64 with every level launched as a full-grid loop). */
+typedef struct pepsmg_options {
+  int32_t smoother;        /* 0: red/black Gauss-Seidel/SOR (reference), 1: CG iterator */
+  int32_t precision;       /* 0: FP64; 1: FP32 V-cycles inside FP64 iterative refinement */
+  int32_t coarse_switch;   /* levels with at most this many cells run inside one
+                              persistent workgroup (0 = off); default 4096 (16^3) */
+  int32_t inner_cycles;    /* precision 1: FP32 V-cycles per refinement step (default 4) */
+  int32_t max_refine;      /* precision 1: cap on refinement steps (default 30) */
+  int32_t fused_smoother;  /* 1 (default): single-pass out-of-place red+black kernel; 0: two colour sweeps */
+  int32_t tile_y;          /* fused smoother tile height in cells (0 = autotuned default) */
+  int32_t verbose;         /* 1: print the reference's [MG] cycle lines */
+} pepsmg_options;
+
+/* Per-solve statistics (the reference only prints these, src/multigrid.f90:1972). */
+typedef struct pepsmg_stats {
+  int32_t cycles;          /* V-cycles run (FP64) or FP32 V-cycles run (precision 1) */
+  int32_t refine_steps;    /* precision 1: refinement steps                      */
+  int32_t converged;       /* 1 if rho < tolerance                               */
+  int32_t history_cap;     /* capacity of history / coarse_iters (caller-owned)  */
+  double res0;             /* |b - A x0|^2                                       */
+  double rho;              /* final sqrt(|b - A x|^2 / res0)                     */
+  double *history;         /* rho after each V-cycle (FP64) or refinement step   */
+  int32_t *coarse_iters;   /* coarsest-level solver iterations per V-cycle       */
+  double device_ms;        /* device time of the solve (events on the solve stream) */
+} pepsmg_stats;
+
+typedef struct pepsmg_ctx pepsmg_ctx;
+
+/* Last error message of the calling thread ("" if none). */
+const char *pepsmg_last_error(void);
+
+/* Default options into *opt. */
+void pepsmg_default_options(pepsmg_options *opt);
+
+/* multigrid_create (src/multigrid.f90:75-159): builds the level hierarchy
+ * (sizes halve per axis while even; per-level metrics as
+ * multigrid_subdomain_CGS_make_grid, :421-495) and allocates device memory.
+ * aggr_method 0/1/2 = CGS/CGA/CGPSA; on one process the three give the same
+ * values, and the reference's checks on nlevel, CGA_level and level counts
+ * (:101-107, :120-124, :191-195, :307-314, :530-534, :545-552, :726-733,
+ * :1186-1193) are applied and reported as PEPSMG_ERR_LEVELS. */
+int pepsmg_multigrid_create(const pepsmg_grid *grid, int32_t nlevel, int32_t ncycle,
+                            int32_t aggr_method, int32_t single_aggr_level,
+                            const int32_t adaptive_aggr_level[3], pepsmg_ctx **out);
+
+int pepsmg_set_options(pepsmg_ctx *ctx, const pepsmg_options *opt);
+
+/* multigrid_solve_vcycle (src/multigrid.f90:1864-1894, CGS cycle
+ * :1907-1976): V(nu,nu) cycles from the initial guess in sol until
+ * sqrt(|r|^2/|r0|^2) < tolerance or ncycle cycles.  On return sol holds the
+ * solution and rsd the final residual b - A sol (ghosts zero).  Host arrays. */
+int pepsmg_multigrid_solve_vcycle(pepsmg_ctx *ctx, double *sol, double *rsd, const double *coeff,
+                                  const double *rhs, int32_t maxiteration, double tolerance,
+                                  double omega_sor, pepsmg_stats *stats);
+
+/* Same on device arrays (padded layout) on the given HIP stream (NULL: the
+ * context's own stream).  rsd may be NULL. */
+int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx *ctx, double *d_sol, double *d_rsd,
+                                         const double *d_rhs, int32_t maxiteration,
+                                         double tolerance, double omega_sor, void *stream,
+                                         pepsmg_stats *stats);
+
+/* multigrid_destroy (src/multigrid.f90:1320-1334). */
+void pepsmg_multigrid_destroy(pepsmg_ctx *ctx);
+
+/* rbgs_iterator_poisson_matrix (src/rbgs_poisson_matrix.f90:191-279):
+ * maxiteration red/black iterations in place on sol.  Host arrays. */
+int pepsmg_rbgs_iterator(const pepsmg_grid *grid, double *sol, const double *coeff,
+                         const double *rhs, int32_t maxiteration, double omega);
+
+/* rbgs_solver_poisson_matrix (src/rbgs_poisson_matrix.f90:47-179): iterates
+ * until sqrt(|r|^2/|r0|^2) <= tolerance or maxiteration iterations; *iters
+ * receives the reference's loop variable on exit, *rel the last ratio. */
+int pepsmg_rbgs_solver(const pepsmg_grid *grid, double *sol, const double *coeff,
+                       const double *rhs, int32_t maxiteration, double tolerance, double omega,
+                       int32_t *iters, double *rel);
+
+/* cg_iterator_poisson_matrix (src/cg_poisson_matrix.f90:157-241). */
+int pepsmg_cg_iterator(const pepsmg_grid *grid, double *sol, const double *coeff,
+                       const double *rhs, int32_t maxiteration);
+
+/* cg_solver_poisson_matrix (src/cg_poisson_matrix.f90:47-146): stops when
+ * sqrt(|r|^2/|r0|^2) < tolerance. */
+int pepsmg_cg_solver(const pepsmg_grid *grid, double *sol, const double *coeff,
+                     const double *rhs, int32_t maxiteration, double tolerance, int32_t *iters,
+                     double *rel);
+
+/* mv_mul_poisson_matrix (src/poisson_matrix_operator.f90:110-161):
+ * y = A x on the interior, y's ghosts set to zero. */
+int pepsmg_mv_mul(const pepsmg_grid *grid, double *y, const double *coeff, const double *x);
+
+/* vv_dot_3d_matrix (src/poisson_matrix_operator.f90:40-99): interior dot
+ * product of two padded arrays. */
+int pepsmg_vv_dot(int32_t nx, int32_t ny, int32_t nz, const double *x, const double *y,
+                  double *result);
+
+/* Measurement hook (no counterpart in the reference): launches the
+ * fine-level smoother kernel `reps` times on the context's level-0 arrays
+ * (precision 0: FP64, 1: FP32) on the context's stream, timing them with
+ * HIP events on that stream.  *avg_ms receives the mean duration of one
+ * launch, *bytes the algorithmic HBM bytes of one launch (each input read
+ * once, each output written once), *kernel_name the kernel's symbol. */
+int pepsmg_bench_smoother(pepsmg_ctx *ctx, int32_t precision, int32_t reps, double omega,
+                          double *avg_ms, double *bytes, const char **kernel_name);
+
+/* Synthetic workload input (no counterpart in the reference, which has no
+ * random right-hand side): writes into a padded device array the field
+ *   f(i,j,k) = scale_rand * (2u-1) + scale_sin * sin(pi x) sin(pi y) sin(pi z)
+ * where u is the 53-bit uniform of SplitMix64(seed ^ lin),
+ * lin = (i-1) + nx*((j-1) + ny*(k-1)), and x,y,z are the cell centres
+ * xg(i), yg(j), zg(k) (host arrays of length n+2); ghosts are set to zero. */
+int pepsmg_fill_rhs_device(int32_t nx, int32_t ny, int32_t nz, double *d_out, uint64_t seed,
+                           double scale_rand, double scale_sin, const double *xg,
+                           const double *yg, const double *zg, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PEPSMG_H */

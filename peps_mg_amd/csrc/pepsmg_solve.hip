@@ -1,0 +1,836 @@
+// pepsmg_solve.hip -- kernel launchers, the V-cycle schedule (FP64 and FP32
+// inside FP64 iterative refinement), the CG smoother and the solve /
+// standalone-operator entry points of the C-ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/pepsmg.h"
+#include "pepsmg_internal.h"
+
+namespace pepsmg {
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+static inline dim3 grid3(int nx, int ny, int nz, int bx, int by) {
+  return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)((ny + by - 1) / by), (unsigned)nz);
+}
+
+template <typename T>
+struct Ops {
+  hipStream_t s;
+
+  void smooth_colour(DevLevel<T>& d, const T* b, T* x, T omega, int colour, bool zero) {
+    int hx = (d.nx + 1) / 2;
+    dim3 g = grid3(hx, d.ny, d.nz, 64, 4), blk(64, 4, 1);
+    if (zero)
+      hipLaunchKernelGGL((k_rb_colour<T, true>), g, blk, 0, s, x, b, d.L(), omega, colour);
+    else
+      hipLaunchKernelGGL((k_rb_colour<T, false>), g, blk, 0, s, x, b, d.L(), omega, colour);
+  }
+
+  // nu red/black iterations on (x, b); zero_start: x is taken as 0 on entry
+  void rbgs(DevLevel<T>& d, T* x, const T* b, int nu, T omega, bool zero_start) {
+    if (nu <= 0) {
+      if (zero_start) hipMemsetAsync(x, 0, d.ncell * sizeof(T), s);
+      return;
+    }
+    for (int it = 0; it < nu; it++) {
+      smooth_colour(d, b, x, omega, 0, zero_start && it == 0);
+      smooth_colour(d, b, x, omega, 1, false);
+    }
+  }
+
+  void residual(DevLevel<T>& d, T* r, const T* x, const T* b) {
+    hipLaunchKernelGGL((k_residual<T>), grid3(d.nx, d.ny, d.nz, 64, 4), dim3(64, 4, 1), 0, s, r, x, b, d.L());
+  }
+
+  void apply(DevLevel<T>& d, T* y, const T* x) {
+    hipLaunchKernelGGL((k_apply<T>), grid3(d.nx, d.ny, d.nz, 64, 4), dim3(64, 4, 1), 0, s, y, x, d.L());
+  }
+
+  void residual_restrict(DevLevel<T>& f, DevLevel<T>& c, const T* x, const T* b) {
+    hipLaunchKernelGGL((k_restrict<T, true>), grid3(c.nx, c.ny, c.nz, 64, 4), dim3(64, 4, 1), 0, s, c.b, x, b,
+                       f.L(), c.L(), f.RW());
+  }
+
+  void prolong_add(DevLevel<T>& f, DevLevel<T>& c, T* xf) {
+    hipLaunchKernelGGL((k_prolong_add<T>), grid3(f.nx, f.ny, f.nz, 64, 4), dim3(64, 4, 1), 0, s, xf, c.x, f.L(),
+                       c.L(), f.PW());
+  }
+
+  int red_blocks(const DevLevel<T>& d) {
+    long n = d.cells();
+    long b = (n + RED_THREADS - 1) / RED_THREADS;
+    return (int)std::max(1L, std::min<long>(b, RED_BLOCKS));
+  }
+
+  // out[0] = sum x*y over the interior
+  void dot(DevLevel<T>& d, const T* x, const T* y, double* partial, double* out) {
+    int nb = red_blocks(d);
+    hipLaunchKernelGGL((k_dot_partial<T, 0, false>), dim3(nb), dim3(RED_THREADS), 0, s, x, y, d.L(), partial,
+                       (T*)nullptr);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+  }
+
+  // out[0] = |b - A x|^2; optionally r = b - A x
+  void resid_norm(DevLevel<T>& d, const T* x, const T* b, T* r, double* partial, double* out) {
+    int nb = red_blocks(d);
+    if (r)
+      hipLaunchKernelGGL((k_dot_partial<T, 1, true>), dim3(nb), dim3(RED_THREADS), 0, s, x, b, d.L(), partial, r);
+    else
+      hipLaunchKernelGGL((k_dot_partial<T, 1, false>), dim3(nb), dim3(RED_THREADS), 0, s, x, b, d.L(), partial,
+                         (T*)nullptr);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// CG iterator (src/cg_poisson_matrix.f90:157-241) on device arrays.  The
+// scalars live on the device: sc[0] = (r,r) at the start of the iteration
+// (temp1), sc[1] = (Ap,p), sc[2] = (r,r) after the update (temp2).
+template <typename T>
+__global__ void k_cg_init(T* __restrict__ r, T* __restrict__ p, const T* __restrict__ x, const T* __restrict__ b,
+                          Lvl<T> L, bool zero_x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > L.nx || j > L.ny) return;
+  long q = lidx(L, i, j, k);
+  T v;
+  if (zero_x) {
+    // A*0 evaluated in the reference order is +0, and b - 0 = b
+    v = b[q] - T(0);
+  } else {
+    v = resid_at(x, b, L, i, j, k);
+  }
+  r[q] = v;
+  p[q] = v;
+}
+
+template <typename T>
+__global__ void k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap,
+                            const double* __restrict__ sc, Lvl<T> L, bool zero_x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > L.nx || j > L.ny) return;
+  long q = lidx(L, i, j, k);
+  T alpha = (T)(sc[0] / sc[1]);                      // :209
+  T xo = zero_x ? T(0) : x[q];
+  x[q] = xo + alpha * p[q];                          // :215
+  r[q] = r[q] - alpha * ap[q];                       // :216
+}
+
+template <typename T>
+__global__ void k_cg_pupdate(T* __restrict__ p, const T* __restrict__ r, double* __restrict__ sc, Lvl<T> L) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  int k = blockIdx.z + 1;
+  if (i > L.nx || j > L.ny) return;
+  long q = lidx(L, i, j, k);
+  T beta = (T)(sc[2] / sc[0]);                       // :223
+  p[q] = r[q] + beta * p[q];                         // :229
+}
+
+__global__ void k_cg_roll(double* sc) { sc[0] = sc[2]; }
+
+template <typename T>
+static void cg_iterate(Ops<T>& op, DevLevel<T>& d, T* x, const T* b, T* r, T* p, T* ap, int nu, bool zero_x,
+                       double* partial, double* sc) {
+  dim3 g = grid3(d.nx, d.ny, d.nz, 64, 4), blk(64, 4, 1);
+  hipLaunchKernelGGL((k_cg_init<T>), g, blk, 0, op.s, r, p, x, b, d.L(), zero_x);
+  if (nu <= 0) {
+    if (zero_x) hipMemsetAsync(x, 0, d.ncell * sizeof(T), op.s);
+    return;
+  }
+  op.dot(d, r, r, partial, sc + 0);                                 // temp1 (:204)
+  for (int it = 0; it < nu; it++) {
+    op.apply(d, ap, p);                                             // :206
+    op.dot(d, ap, p, partial, sc + 1);                              // :207
+    hipLaunchKernelGGL((k_cg_update<T>), g, blk, 0, op.s, x, r, p, ap, sc, d.L(), zero_x && it == 0);
+    op.dot(d, r, r, partial, sc + 2);                               // :221
+    hipLaunchKernelGGL((k_cg_pupdate<T>), g, blk, 0, op.s, p, r, sc, d.L());
+    // the next iteration's temp1 = (r,r) of the same r: the deterministic
+    // reduction gives exactly sc[2]
+    hipLaunchKernelGGL(k_cg_roll, dim3(1), dim3(1), 0, op.s, sc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision helpers
+__global__ void k_f64_to_f32(float* __restrict__ dst, const double* __restrict__ src, long n) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) dst[c] = (float)src[c];
+}
+__global__ void k_f32_to_f64(double* __restrict__ dst, const float* __restrict__ src, long n) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) dst[c] = (double)src[c];
+}
+
+// x_out = x + e; r32 = float(b - A x_out); partial sums of (b - A x_out)^2.
+// Out of place: neighbours of x_out are formed on the fly from x and e.
+__global__ void __launch_bounds__(RED_THREADS) k_refine_update(double* __restrict__ xo, const double* __restrict__ x,
+                                                               const float* __restrict__ e, const double* __restrict__ b,
+                                                               float* __restrict__ r32, Lvl<double> L,
+                                                               double* __restrict__ partial, int first) {
+  __shared__ double sh[RED_THREADS / 64];
+  long total = (long)L.nx * L.ny * L.nz;
+  double acc = 0.0;
+  auto X = [&](int i, int j, int k) -> double {
+    if (i < 1 || i > L.nx || j < 1 || j > L.ny || k < 1 || k > L.nz) return 0.0;
+    long q = lidx(L, i, j, k);
+    return first ? x[q] : x[q] + (double)e[q];
+  };
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
+    int i = (int)(c % L.nx) + 1;
+    long row = c / L.nx;
+    int j = (int)(row % L.ny) + 1;
+    int k = (int)(row / L.ny) + 1;
+    double c1 = L.cwx[i], c2 = L.cex[i], c3 = L.cwy[j], c4 = L.cey[j], c5 = L.cwz[k], c6 = L.cez[k];
+    double c0 = diag(c1, c2, c3, c4, c5, c6);
+    double xc = X(i, j, k);
+    double y = c0 * xc + c1 * X(i - 1, j, k) + c2 * X(i + 1, j, k) + c3 * X(i, j - 1, k) + c4 * X(i, j + 1, k) +
+               c5 * X(i, j, k - 1) + c6 * X(i, j, k + 1);
+    long q = lidx(L, i, j, k);
+    double r = b[q] - y;
+    if (!first) xo[q] = xc;
+    r32[q] = (float)r;
+    acc += r * r;
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+}  // namespace pepsmg
+
+using namespace pepsmg;
+
+// ---------------------------------------------------------------------------
+int pepsmg_ctx::alloc_scratch() {
+  HIP_TRY(hipMalloc(&d_partial, RED_BLOCKS * sizeof(double)));
+  HIP_TRY(hipMalloc(&d_scalars, 64 * sizeof(double)));
+  HIP_TRY(hipMalloc(&d_flag, sizeof(int)));
+  citers_cap = std::max(ncycle, 1) + 64;
+  HIP_TRY(hipMalloc(&d_citers, citers_cap * sizeof(int)));
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  return PEPSMG_OK;
+}
+
+void pepsmg_ctx::release() {
+  cg64.release();
+  cg32.release();
+  h64.release();
+  h32.release();
+  hipFree(d_partial);
+  hipFree(d_scalars);
+  hipFree(d_flag);
+  hipFree(d_citers);
+  hipFree(d_pad);
+  hipFree(d_pad2);
+  d_partial = d_scalars = d_pad = d_pad2 = nullptr;
+  d_flag = d_citers = nullptr;
+  if (ev0) hipEventDestroy(ev0);
+  if (ev1) hipEventDestroy(ev1);
+  ev0 = ev1 = nullptr;
+  if (stream) hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
+namespace pepsmg {
+
+// Smoothing step on level l of hierarchy h (rbgs_iterator or cg_iterator).
+template <typename T>
+static void smooth(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int l, T* x, const T* b, int nu,
+                   T omega, bool zero_start) {
+  DevLevel<T>& d = h.lv[l];
+  if (c->opt.smoother == 1)
+    cg_iterate(op, d, x, b, d.r, cg->p[l], cg->ap[l], nu, zero_start, c->d_partial, c->d_scalars + 8);
+  else
+    op.rbgs(d, x, b, nu, omega, zero_start);
+}
+
+// Coarsest level: FP64 solve in one workgroup (src/multigrid.f90:1944-1949,
+// tolerance = the V-cycle tolerance, at most 1000 iterations).
+static void coarse_solve64(pepsmg_ctx* c, DevLevel<double>& d, double omega, double tol, int* slot) {
+  hipLaunchKernelGGL((k_coarse_solve<double>), dim3(1), dim3(COARSE_THREADS), 0, c->stream, d.x, d.b, d.L(),
+                     omega, tol, 1000, (int*)nullptr, slot);
+}
+
+// The down/up legs of one V-cycle below the fine level, for hierarchy h
+// whose level-1 right-hand side has been restricted already
+// (src/multigrid.f90:1936-1962).  In FP32 the coarsest solve runs in FP64.
+template <typename T>
+static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int nu, T omega, double tol,
+                               int* slot);
+
+template <>
+void vcycle_coarse_part<double>(pepsmg_ctx* c, Ops<double>& op, Hier<double>& h, CgBufs<double>* cg, int nu,
+                                double omega, double tol, int* slot) {
+  int L = c->nlevel;
+  for (int l = 1; l <= L - 1; l++) {
+    DevLevel<double>& d = h.lv[l];
+    smooth(c, op, h, cg, l, d.x, d.b, nu, omega, true);
+    op.residual_restrict(d, h.lv[l + 1], d.x, d.b);
+  }
+  coarse_solve64(c, h.lv[L], omega, tol, slot);
+  for (int l = L - 1; l >= 1; l--) {
+    DevLevel<double>& d = h.lv[l];
+    op.prolong_add(d, h.lv[l + 1], d.x);
+    smooth(c, op, h, cg, l, d.x, d.b, nu, omega, false);
+  }
+}
+
+template <>
+void vcycle_coarse_part<float>(pepsmg_ctx* c, Ops<float>& op, Hier<float>& h, CgBufs<float>* cg, int nu,
+                               float omega, double tol, int* slot) {
+  int L = c->nlevel;
+  for (int l = 1; l <= L - 1; l++) {
+    DevLevel<float>& d = h.lv[l];
+    smooth(c, op, h, cg, l, d.x, d.b, nu, omega, true);
+    op.residual_restrict(d, h.lv[l + 1], d.x, d.b);
+  }
+  DevLevel<float>& cf = h.lv[L];
+  DevLevel<double>& cd = c->h64.lv[L];
+  long n = (long)cf.ncell;
+  hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, op.s, cd.b, cf.b, n);
+  coarse_solve64(c, cd, (double)omega, tol, slot);
+  hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, op.s, cf.x, cd.x, n);
+  for (int l = L - 1; l >= 1; l--) {
+    DevLevel<float>& d = h.lv[l];
+    op.prolong_add(d, h.lv[l + 1], d.x);
+    smooth(c, op, h, cg, l, d.x, d.b, nu, omega, false);
+  }
+}
+
+static void print_cycle(int cyc, double rv, double r0) {
+  std::printf("[MG] cycle = %4d, Error: %15.7E %15.7E %15.7E\n", cyc, std::sqrt(rv), std::sqrt(r0),
+              std::sqrt(rv / r0));
+}
+
+// FP64 V-cycles (src/multigrid.f90:1907-1976) on level-0 arrays already in
+// h64.lv[0].x / .b.
+static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_stats* st, std::vector<double>& hist,
+                      std::vector<int>& cit, double* res0_out, int* ncyc_out, double* rho_out) {
+  Hier<double>& h = c->h64;
+  Ops<double> op{c->stream};
+  CgBufs<double>* cg = nullptr;
+  if (c->opt.smoother == 1) {
+    cg = &c->cg64;
+    int rc = cg->ensure(h);
+    if (rc) return rc;
+  }
+  DevLevel<double>& f = h.lv[0];
+  double hs[2];
+  op.resid_norm(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 0);      // :1924-1925
+  HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double res0 = hs[0];
+  *res0_out = res0;
+  int ncyc = 0;
+  double rho = 1.0;
+  for (int cyc = 1; cyc <= c->ncycle; cyc++) {
+    smooth(c, op, h, cg, 0, f.x, f.b, nu, omega, false);                   // :1931
+    op.residual_restrict(f, h.lv[1], f.x, f.b);                            // :1932-1933
+    int* slot = (cyc - 1 < c->citers_cap) ? c->d_citers + (cyc - 1) : nullptr;
+    vcycle_coarse_part<double>(c, op, h, cg, nu, omega, tol, slot);        // :1936-1962
+    op.prolong_add(f, h.lv[1], f.x);                                       // :1965-1966
+    smooth(c, op, h, cg, 0, f.x, f.b, nu, omega, false);                   // :1967
+    op.resid_norm(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 1);   // :1970-1971
+    HIP_TRY(hipMemcpyAsync(hs, c->d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    rho = std::sqrt(hs[0] / res0);
+    hist.push_back(rho);
+    ncyc = cyc;
+    if (c->opt.verbose) print_cycle(cyc, hs[0], res0);
+    if (rho < tol) break;                                                  // :1973
+  }
+  if (ncyc > 0) {
+    int m = std::min(ncyc, c->citers_cap);
+    cit.resize(m);
+    HIP_TRY(hipMemcpy(cit.data(), c->d_citers, m * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  *ncyc_out = ncyc;
+  *rho_out = rho;
+  (void)st;
+  return PEPSMG_OK;
+}
+
+// FP32 V-cycles inside FP64 iterative refinement.  The fine residual
+// b - A x, the update x += e and the coarsest solve stay in FP64.
+static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vector<double>& hist,
+                       std::vector<int>& cit, double* res0_out, int* ncyc_out, int* nref_out, double* rho_out) {
+  Hier<double>& h = c->h64;
+  Hier<float>& g = c->h32;
+  Ops<float> op{c->stream};
+  CgBufs<float>* cg = nullptr;
+  if (c->opt.smoother == 1) {
+    cg = &c->cg32;
+    int rc = cg->ensure(g);
+    if (rc) return rc;
+  }
+  DevLevel<double>& f = h.lv[0];
+  DevLevel<float>& f32 = g.lv[0];
+  int nb = Ops<double>{c->stream}.red_blocks(f);
+  double hs[1];
+  // r32 = b - A x; res0
+  hipLaunchKernelGGL(k_refine_update, dim3(nb), dim3(RED_THREADS), 0, c->stream, f.x2, f.x, f32.x, f.b, f32.b, f.L(),
+                     c->d_partial, 1);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
+  HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double res0 = hs[0];
+  *res0_out = res0;
+  int ncyc = 0, nref = 0;
+  double rho = 1.0;
+  float om = (float)omega;
+  std::vector<int> cyc_it;
+  for (int s = 1; s <= c->opt.max_refine && ncyc < c->ncycle; s++) {
+    for (int q = 0; q < c->opt.inner_cycles; q++) {
+      int* slot = (ncyc < c->citers_cap) ? c->d_citers + ncyc : nullptr;
+      smooth(c, op, g, cg, 0, f32.x, f32.b, nu, om, q == 0);
+      op.residual_restrict(f32, g.lv[1], f32.x, f32.b);
+      vcycle_coarse_part<float>(c, op, g, cg, nu, om, tol, slot);
+      op.prolong_add(f32, g.lv[1], f32.x);
+      smooth(c, op, g, cg, 0, f32.x, f32.b, nu, om, false);
+      ncyc++;
+    }
+    // x2 = x + e; r32 = float(b - A x2); |b - A x2|^2
+    hipLaunchKernelGGL(k_refine_update, dim3(nb), dim3(RED_THREADS), 0, c->stream, f.x2, f.x, f32.x, f.b, f32.b,
+                       f.L(), c->d_partial, 0);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
+    std::swap(f.x, f.x2);
+    HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    rho = std::sqrt(hs[0] / res0);
+    hist.push_back(rho);
+    nref = s;
+    if (c->opt.verbose) print_cycle(s, hs[0], res0);
+    if (rho < tol) break;
+  }
+  if (ncyc > 0) {
+    int m = std::min(ncyc, c->citers_cap);
+    cit.resize(m);
+    HIP_TRY(hipMemcpy(cit.data(), c->d_citers, m * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  *ncyc_out = ncyc;
+  *nref_out = nref;
+  *rho_out = rho;
+  return PEPSMG_OK;
+}
+
+template <typename T>
+static void pack(hipStream_t s, T* dst, const double* src, const DevLevel<T>& d, int* flag) {
+  hipLaunchKernelGGL((k_pack<T>), grid3(d.nx + 2, d.ny + 2, d.nz + 2, 64, 4), dim3(64, 4, 1), 0, s, dst, src, d.L(),
+                     flag);
+}
+template <typename T>
+static void unpack(hipStream_t s, double* dst, const T* src, const DevLevel<T>& d, int zero_ghosts) {
+  hipLaunchKernelGGL((k_unpack<T>), grid3(d.nx + 2, d.ny + 2, d.nz + 2, 64, 4), dim3(64, 4, 1), 0, s, dst, src,
+                     d.L(), zero_ghosts);
+}
+
+// Compare a reference AoS matrix with the metric rediscretisation.
+static int check_matrix(const HostLevel& h, const double* coeff) {
+  if (!coeff) return PEPSMG_OK;
+  int nx = h.n[0], ny = h.n[1], nz = h.n[2];
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        const double* q = coeff + 7 * ((size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k - 1)));
+        double c1 = h.cw[0][i], c2 = h.ce[0][i], c3 = h.cw[1][j], c4 = h.ce[1][j], c5 = h.cw[2][k], c6 = h.ce[2][k];
+        double c0 = -(c1 + c2 + c3 + c4 + c5 + c6);
+        if (q[0] != c0 || q[1] != c1 || q[2] != c2 || q[3] != c3 || q[4] != c4 || q[5] != c5 || q[6] != c6)
+          return fail(PEPSMG_ERR_MATRIX,
+                      "coefficient matrix differs from the rediscretisation of the grid metrics at (%d,%d,%d)", i, j, k);
+      }
+  return PEPSMG_OK;
+}
+
+static int check_ghosts_host(const double* a, int nx, int ny, int nz, const char* name) {
+  size_t sx = nx + 2, sy = ny + 2;
+  for (int k = 0; k <= nz + 1; k++)
+    for (int j = 0; j <= ny + 1; j++) {
+      bool full = (k == 0 || k == nz + 1 || j == 0 || j == ny + 1);
+      const double* row = a + sx * ((size_t)j + sy * k);
+      if (full) {
+        for (int i = 0; i <= nx + 1; i++)
+          if (row[i] != 0.0) return fail(PEPSMG_ERR_GHOST, "%s has a non-zero ghost cell at (%d,%d,%d)", name, i, j, k);
+      } else if (row[0] != 0.0 || row[nx + 1] != 0.0) {
+        return fail(PEPSMG_ERR_GHOST, "%s has a non-zero ghost cell in row (%d,%d)", name, j, k);
+      }
+    }
+  return PEPSMG_OK;
+}
+
+static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_stats* st) {
+  std::vector<double> hist;
+  std::vector<int> cit;
+  double res0 = 0, rho = 0;
+  int ncyc = 0, nref = 0, rc;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  if (c->opt.precision == 1)
+    rc = solve_mixed(c, nu, tol, omega, hist, cit, &res0, &ncyc, &nref, &rho);
+  else
+    rc = solve_fp64(c, nu, tol, omega, st, hist, cit, &res0, &ncyc, &rho);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  if (st) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    st->device_ms = ms;
+    st->cycles = ncyc;
+    st->refine_steps = nref;
+    st->res0 = res0;
+    st->rho = rho;
+    st->converged = rho < tol ? 1 : 0;
+    if (st->history)
+      for (int q = 0; q < (int)hist.size() && q < st->history_cap; q++) st->history[q] = hist[q];
+    if (st->coarse_iters)
+      for (int q = 0; q < (int)cit.size() && q < st->history_cap; q++) st->coarse_iters[q] = cit[q];
+  }
+  return PEPSMG_OK;
+}
+
+}  // namespace pepsmg
+
+extern "C" {
+
+int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d_rsd, const double* d_rhs,
+                                         int32_t maxiteration, double tolerance, double omega_sor, void* stream,
+                                         pepsmg_stats* st) {
+  if (!c || !d_sol || !d_rhs) return fail(PEPSMG_ERR_ARG, "NULL argument");
+  if (maxiteration < 0) return fail(PEPSMG_ERR_ARG, "maxiteration < 0");
+  hipStream_t saved = c->stream;
+  if (stream) c->stream = (hipStream_t)stream;
+  DevLevel<double>& f = c->h64.lv[0];
+  int rc = PEPSMG_OK;
+  int flag = 0;
+  hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
+  pack(c->stream, f.x, d_sol, f, c->d_flag);
+  pack(c->stream, f.b, d_rhs, f, (int*)nullptr);
+  hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) rc = hip_fail(e, "pack");
+  else if (flag) rc = fail(PEPSMG_ERR_GHOST, "sol has non-zero ghost cells (homogeneous Dirichlet required)");
+  if (!rc) rc = run_solve(c, maxiteration, tolerance, omega_sor, st);
+  if (!rc) {
+    unpack(c->stream, d_sol, f.x, f, 0);
+    if (d_rsd) {
+      Ops<double> op{c->stream};
+      op.residual(f, f.r, f.x, f.b);                                   // :1970
+      unpack(c->stream, d_rsd, f.r, f, 1);
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "unpack");
+  }
+  c->stream = saved;
+  return rc;
+}
+
+int pepsmg_multigrid_solve_vcycle(pepsmg_ctx* c, double* sol, double* rsd, const double* coeff, const double* rhs,
+                                  int32_t maxiteration, double tolerance, double omega_sor, pepsmg_stats* st) {
+  if (!c || !sol || !rhs) return fail(PEPSMG_ERR_ARG, "NULL argument");
+  const HostLevel& h = c->host[0];
+  int rc = check_matrix(h, coeff);
+  if (rc) return rc;
+  rc = check_ghosts_host(sol, h.n[0], h.n[1], h.n[2], "sol");
+  if (rc) return rc;
+  size_t np = (size_t)(h.n[0] + 2) * (h.n[1] + 2) * (h.n[2] + 2);
+  if (c->pad_elems < np) {
+    hipFree(c->d_pad);
+    hipFree(c->d_pad2);
+    c->d_pad = c->d_pad2 = nullptr;
+    HIP_TRY(hipMalloc(&c->d_pad, np * sizeof(double)));
+    HIP_TRY(hipMalloc(&c->d_pad2, np * sizeof(double)));
+    c->pad_elems = np;
+  }
+  HIP_TRY(hipMemcpy(c->d_pad, sol, np * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_pad2, rhs, np * sizeof(double), hipMemcpyHostToDevice));
+  // rhs staging is reused for rsd after the solve
+  rc = pepsmg_multigrid_solve_vcycle_device(c, c->d_pad, nullptr, c->d_pad2, maxiteration, tolerance, omega_sor,
+                                            nullptr, st);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(sol, c->d_pad, np * sizeof(double), hipMemcpyDeviceToHost));
+  if (rsd) {
+    DevLevel<double>& f = c->h64.lv[0];
+    Ops<double> op{c->stream};
+    op.residual(f, f.r, f.x, f.b);
+    unpack(c->stream, c->d_pad2, f.r, f, 1);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(rsd, c->d_pad2, np * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return PEPSMG_OK;
+}
+
+int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double omega, double* avg_ms,
+                          double* bytes, const char** kernel_name) {
+  if (!c || reps < 1) return fail(PEPSMG_ERR_ARG, "invalid argument");
+  if (precision == 1 && c->h32.lv.empty()) return fail(PEPSMG_ERR_ARG, "FP32 hierarchy not built");
+  float ms = 0;
+  double n = 0, es = precision == 1 ? 4.0 : 8.0;
+  const char* name = "";
+  // one colour sweep of the two-kernel smoother: reads x (all cells) and b,
+  // writes half of x: 2 n elements per launch
+  auto body = [&](auto& op, auto& d) {
+    for (int q = 0; q < reps; q++) op.smooth_colour(d, d.b, d.x, (decltype(d.x[0] + 0))omega, q & 1, false);
+  };
+  if (precision == 1) {
+    Ops<float> op{c->stream};
+    DevLevel<float>& d = c->h32.lv[0];
+    body(op, d);  // warm
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    body(op, d);
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    n = (double)d.cells();
+    name = "k_rb_colour<float, false>";
+  } else {
+    Ops<double> op{c->stream};
+    DevLevel<double>& d = c->h64.lv[0];
+    body(op, d);
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    body(op, d);
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    n = (double)d.cells();
+    name = "k_rb_colour<double, false>";
+  }
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (avg_ms) *avg_ms = ms / reps;
+  if (bytes) *bytes = 2.0 * n * es;
+  if (kernel_name) *kernel_name = name;
+  return PEPSMG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Standalone operators on one grid (host arrays).
+namespace pepsmg {
+
+struct OneLevel {
+  HostLevel h;
+  DevLevel<double> d;
+  double *pad_a = nullptr, *pad_b = nullptr, *partial = nullptr, *sc = nullptr;
+  int* flag = nullptr;
+  hipStream_t s = nullptr;
+  size_t np = 0;
+  ~OneLevel() {
+    hipFree(d.cw[0]); hipFree(d.cw[1]); hipFree(d.cw[2]);
+    hipFree(d.ce[0]); hipFree(d.ce[1]); hipFree(d.ce[2]);
+    hipFree(d.x); hipFree(d.b); hipFree(d.r); hipFree(d.x2);
+    hipFree(pad_a); hipFree(pad_b); hipFree(partial); hipFree(sc); hipFree(flag);
+    if (s) hipStreamDestroy(s);
+  }
+  int init(const pepsmg_grid* g) {
+    if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1 || !g->dxm || !g->dxg || !g->dym || !g->dyg || !g->dzm || !g->dzg)
+      return fail(PEPSMG_ERR_ARG, "invalid grid descriptor");
+    h.n[0] = g->nx; h.n[1] = g->ny; h.n[2] = g->nz;
+    const double* dms[3] = {g->dxm, g->dym, g->dzm};
+    const double* dgs[3] = {g->dxg, g->dyg, g->dzg};
+    for (int a = 0; a < 3; a++) {
+      h.dm[a].assign(dms[a], dms[a] + h.n[a] + 2);
+      h.dg[a].assign(dgs[a], dgs[a] + h.n[a] + 2);
+    }
+    h.setup_coeffs();
+    d.nx = h.n[0]; d.ny = h.n[1]; d.nz = h.n[2];
+    d.px = pitch_for(d.nx);
+    d.pxy = d.px * d.ny;
+    d.ncell = (size_t)d.pxy * d.nz;
+    for (int a = 0; a < 3; a++) {
+      HIP_TRY(hipMalloc(&d.cw[a], h.cw[a].size() * sizeof(double)));
+      HIP_TRY(hipMalloc(&d.ce[a], h.ce[a].size() * sizeof(double)));
+      HIP_TRY(hipMemcpy(d.cw[a], h.cw[a].data(), h.cw[a].size() * sizeof(double), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d.ce[a], h.ce[a].data(), h.ce[a].size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMalloc(&d.x, d.ncell * sizeof(double)));
+    HIP_TRY(hipMalloc(&d.b, d.ncell * sizeof(double)));
+    HIP_TRY(hipMalloc(&d.r, d.ncell * sizeof(double)));
+    HIP_TRY(hipMalloc(&d.x2, d.ncell * sizeof(double)));
+    np = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 2);
+    HIP_TRY(hipMalloc(&pad_a, np * sizeof(double)));
+    HIP_TRY(hipMalloc(&pad_b, np * sizeof(double)));
+    HIP_TRY(hipMalloc(&partial, RED_BLOCKS * sizeof(double)));
+    HIP_TRY(hipMalloc(&sc, 16 * sizeof(double)));
+    HIP_TRY(hipMalloc(&flag, sizeof(int)));
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return PEPSMG_OK;
+  }
+  int upload(double* dst, const double* host) {
+    HIP_TRY(hipMemcpy(pad_a, host, np * sizeof(double), hipMemcpyHostToDevice));
+    pack(s, dst, pad_a, d, (int*)nullptr);
+    HIP_TRY(hipStreamSynchronize(s));
+    return PEPSMG_OK;
+  }
+  int download(double* host, const double* src, int zero_ghosts) {
+    HIP_TRY(hipMemcpy(pad_b, host, np * sizeof(double), hipMemcpyHostToDevice));
+    unpack(s, pad_b, src, d, zero_ghosts);
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(host, pad_b, np * sizeof(double), hipMemcpyDeviceToHost));
+    return PEPSMG_OK;
+  }
+  int scalar(int q, double* out) {
+    HIP_TRY(hipMemcpyAsync(out, sc + q, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return PEPSMG_OK;
+  }
+};
+
+// rbgs_solver on one level (src/rbgs_poisson_matrix.f90:73-171): kernel
+// loop with the norm reduced on the device and tested on the host.
+static int rbgs_solver_level(OneLevel& o, int maxit, double tol, double omega, int* iters, double* rel) {
+  Ops<double> op{o.s};
+  op.resid_norm(o.d, o.d.x, o.d.b, nullptr, o.partial, o.sc + 0);
+  double r0, rn = 0;
+  int rc = o.scalar(0, &r0);
+  if (rc) return rc;
+  rn = r0;
+  int it;
+  for (it = 0; it < maxit; it++) {
+    op.rbgs(o.d, o.d.x, o.d.b, 1, omega, false);
+    op.resid_norm(o.d, o.d.x, o.d.b, nullptr, o.partial, o.sc + 1);
+    rc = o.scalar(1, &rn);
+    if (rc) return rc;
+    if (std::sqrt(rn / r0) <= tol) break;
+  }
+  if (iters) *iters = it;
+  if (rel) *rel = std::sqrt(rn / r0);
+  return PEPSMG_OK;
+}
+
+}  // namespace pepsmg
+
+extern "C" {
+
+int pepsmg_rbgs_iterator(const pepsmg_grid* grid, double* sol, const double* coeff, const double* rhs,
+                         int32_t maxiteration, double omega) {
+  OneLevel o;
+  int rc = o.init(grid);
+  if (!rc) rc = check_matrix(o.h, coeff);
+  if (!rc) rc = check_ghosts_host(sol, o.d.nx, o.d.ny, o.d.nz, "sol");
+  if (!rc) rc = o.upload(o.d.x, sol);
+  if (!rc) rc = o.upload(o.d.b, rhs);
+  if (rc) return rc;
+  Ops<double> op{o.s};
+  op.rbgs(o.d, o.d.x, o.d.b, maxiteration, omega, false);
+  return o.download(sol, o.d.x, 0);
+}
+
+int pepsmg_rbgs_solver(const pepsmg_grid* grid, double* sol, const double* coeff, const double* rhs,
+                       int32_t maxiteration, double tolerance, double omega, int32_t* iters, double* rel) {
+  OneLevel o;
+  int rc = o.init(grid);
+  if (!rc) rc = check_matrix(o.h, coeff);
+  if (!rc) rc = check_ghosts_host(sol, o.d.nx, o.d.ny, o.d.nz, "sol");
+  if (!rc) rc = o.upload(o.d.x, sol);
+  if (!rc) rc = o.upload(o.d.b, rhs);
+  if (rc) return rc;
+  int it = 0;
+  double rl = 0;
+  rc = rbgs_solver_level(o, maxiteration, tolerance, omega, &it, &rl);
+  if (rc) return rc;
+  if (iters) *iters = it;
+  if (rel) *rel = rl;
+  return o.download(sol, o.d.x, 0);
+}
+
+int pepsmg_cg_iterator(const pepsmg_grid* grid, double* sol, const double* coeff, const double* rhs,
+                       int32_t maxiteration) {
+  OneLevel o;
+  int rc = o.init(grid);
+  if (!rc) rc = check_matrix(o.h, coeff);
+  if (!rc) rc = check_ghosts_host(sol, o.d.nx, o.d.ny, o.d.nz, "sol");
+  if (!rc) rc = o.upload(o.d.x, sol);
+  if (!rc) rc = o.upload(o.d.b, rhs);
+  if (rc) return rc;
+  Ops<double> op{o.s};
+  double* p = nullptr;
+  double* ap = nullptr;
+  HIP_TRY(hipMalloc(&p, o.d.ncell * sizeof(double)));
+  HIP_TRY(hipMalloc(&ap, o.d.ncell * sizeof(double)));
+  cg_iterate(op, o.d, o.d.x, o.d.b, o.d.r, p, ap, maxiteration, false, o.partial, o.sc);
+  rc = o.download(sol, o.d.x, 0);
+  hipFree(p);
+  hipFree(ap);
+  return rc;
+}
+
+int pepsmg_cg_solver(const pepsmg_grid* grid, double* sol, const double* coeff, const double* rhs,
+                     int32_t maxiteration, double tolerance, int32_t* iters, double* rel) {
+  OneLevel o;
+  int rc = o.init(grid);
+  if (!rc) rc = check_matrix(o.h, coeff);
+  if (!rc) rc = check_ghosts_host(sol, o.d.nx, o.d.ny, o.d.nz, "sol");
+  if (!rc) rc = o.upload(o.d.x, sol);
+  if (!rc) rc = o.upload(o.d.b, rhs);
+  if (rc) return rc;
+  Ops<double> op{o.s};
+  double *p = nullptr, *ap = nullptr;
+  HIP_TRY(hipMalloc(&p, o.d.ncell * sizeof(double)));
+  HIP_TRY(hipMalloc(&ap, o.d.ncell * sizeof(double)));
+  dim3 g = grid3(o.d.nx, o.d.ny, o.d.nz, 64, 4), blk(64, 4, 1);
+  double* sc = o.sc;
+  // src/cg_poisson_matrix.f90:81-137
+  hipLaunchKernelGGL((k_cg_init<double>), g, blk, 0, o.s, o.d.r, p, o.d.x, o.d.b, o.d.L(), false);
+  op.dot(o.d, o.d.r, o.d.r, o.partial, sc + 3);                    // rsd0tol (:94)
+  double r0 = 0, t2 = 0;
+  rc = o.scalar(3, &r0);
+  int it;
+  for (it = 0; it < maxiteration && !rc; it++) {
+    op.dot(o.d, o.d.r, o.d.r, o.partial, sc + 0);                  // :106
+    op.apply(o.d, ap, p);                                          // :108
+    op.dot(o.d, ap, p, o.partial, sc + 1);                         // :109
+    hipLaunchKernelGGL((k_cg_update<double>), g, blk, 0, o.s, o.d.x, o.d.r, p, ap, sc, o.d.L(), false);
+    op.dot(o.d, o.d.r, o.d.r, o.partial, sc + 2);                  // :123
+    rc = o.scalar(2, &t2);
+    if (rc) break;
+    if (std::sqrt(t2 / r0) < tolerance) break;                     // :126
+    hipLaunchKernelGGL((k_cg_pupdate<double>), g, blk, 0, o.s, p, o.d.r, sc, o.d.L());
+  }
+  hipFree(p);
+  hipFree(ap);
+  if (rc) return rc;
+  if (iters) *iters = it;
+  if (rel) *rel = std::sqrt(t2 / r0);
+  return o.download(sol, o.d.x, 0);
+}
+
+int pepsmg_mv_mul(const pepsmg_grid* grid, double* y, const double* coeff, const double* x) {
+  OneLevel o;
+  int rc = o.init(grid);
+  if (!rc) rc = check_matrix(o.h, coeff);
+  if (!rc) rc = check_ghosts_host(x, o.d.nx, o.d.ny, o.d.nz, "x");
+  if (!rc) rc = o.upload(o.d.x, x);
+  if (rc) return rc;
+  Ops<double> op{o.s};
+  op.apply(o.d, o.d.r, o.d.x);
+  return o.download(y, o.d.r, 1);
+}
+
+int pepsmg_vv_dot(int32_t nx, int32_t ny, int32_t nz, const double* x, const double* y, double* result) {
+  if (nx < 1 || ny < 1 || nz < 1 || !x || !y || !result) return fail(PEPSMG_ERR_ARG, "invalid argument");
+  std::vector<double> ones[3];
+  std::vector<double> dm[3], dg[3];
+  int n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; a++) {
+    dm[a].assign(n[a] + 2, 1.0);
+    dg[a].assign(n[a] + 2, 1.0);
+  }
+  pepsmg_grid g{nx, ny, nz, dm[0].data(), dg[0].data(), dm[1].data(), dg[1].data(), dm[2].data(), dg[2].data()};
+  OneLevel o;
+  int rc = o.init(&g);
+  if (!rc) rc = o.upload(o.d.x, x);
+  if (!rc) rc = o.upload(o.d.b, y);
+  if (rc) return rc;
+  Ops<double> op{o.s};
+  op.dot(o.d, o.d.x, o.d.b, o.partial, o.sc);
+  return o.scalar(0, result);
+}
+
+}  // extern "C"

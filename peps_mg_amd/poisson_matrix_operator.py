@@ -1,0 +1,20 @@
+"""Matrix-vector product and dot product (module poisson_matrix_operator,
+src/poisson_matrix_operator.f90) on the GPU."""
+import ctypes as C
+
+from . import _lib
+from .geometry import grid_descriptor
+
+
+def vv_dot_3d_matrix(x, y, nx, ny, nz, is_serial=(True,) * 3):
+    """src/poisson_matrix_operator.f90:40-99; returns the interior dot product."""
+    out = C.c_double(0)
+    _lib.check(_lib.lib().pepsmg_vv_dot(nx, ny, nz, _lib.dptr(x), _lib.dptr(y), C.byref(out)))
+    return out.value
+
+
+def mv_mul_poisson_matrix(y, a_poisson, x, dm, is_serial=(True,) * 3):
+    """src/poisson_matrix_operator.f90:110-161; y written in place (ghosts 0)."""
+    g = grid_descriptor(dm)
+    coeff = None if a_poisson is None else a_poisson.coeff
+    _lib.check(_lib.lib().pepsmg_mv_mul(C.byref(g), _lib.dptr(y), _lib.dptr(coeff), _lib.dptr(x)))

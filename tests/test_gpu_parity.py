@@ -1,0 +1,249 @@
+"""GPU parity: libpepsmg.so (through its C-ABI) against the oracle.
+
+FP64 smoother, residual, transfer and V-cycle values are bit-identical to
+the oracle's (same expressions, no FMA contraction; the coarsest solve sums
+its norms in the reference's order).  Dot products use a tree order, so
+norms and the per-cycle relative residuals agree to rounding (|drho| <= 1e-12).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import refmath as RM
+from oracle import oracle as O
+from peps_mg_amd import _lib, geometry, matrix
+from peps_mg_amd.cg_poisson_matrix import cg_iterator_poisson_matrix, cg_solver_poisson_matrix
+from peps_mg_amd.multigrid import Multigrid
+from peps_mg_amd.poisson_matrix_operator import mv_mul_poisson_matrix, vv_dot_3d_matrix
+from peps_mg_amd.rbgs_poisson_matrix import rbgs_iterator_poisson_matrix, rbgs_solver_poisson_matrix
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(99)
+
+
+def sdm_from(g):
+    """Subdomain carrying exactly the oracle grid's metrics."""
+    s = geometry.Subdomain(g.nx, g.ny, g.nz, 0, 0, 0, g.sox, g.soy, g.soz, g.dxm, g.dym, g.dzm, g.dxg, g.dyg,
+                           g.dzg, g.xg, g.yg, g.zg)
+    return s
+
+
+def rand_padded(nx, ny, nz):
+    a = O.padded(nx, ny, nz)
+    a[1:-1, 1:-1, 1:-1] = RNG.uniform(-1, 1, (nz, ny, nx))
+    return a
+
+
+CASES = [((16, 16, 16), (1, 1, 1), (1, 1, 1)), ((12, 10, 9), (1, 2, 1), (1, 1, 1)),
+         ((16, 8, 16), (1, 1, 1), (1.15, 1.0, 1.2)), ((5, 7, 3), (1, 1, 1), (1, 1, 1))]
+
+
+@pytest.mark.parametrize("dims,lens,stretch", CASES)
+def test_mv_mul_bitwise(dims, lens, stretch):
+    g = O.Grid(*dims, l=lens, a=stretch)
+    A = g.matrix()
+    x = rand_padded(*dims)
+    y = O.padded(*dims)
+    y[0, 0, 0] = 5.0                                    # ghosts are overwritten with 0
+    mv_mul_poisson_matrix(y, matrix.MatrixHeptadiagonal(0, A), x, sdm_from(g))
+    assert np.array_equal(y, O.mv_mul(A, x))
+
+
+def test_vv_dot():
+    x, y = rand_padded(33, 17, 9), rand_padded(33, 17, 9)
+    v = vv_dot_3d_matrix(x, y, 33, 17, 9)
+    assert abs(v - O.vv_dot(x, y)) <= 1e-13 * abs(O.vv_dot(np.abs(x), np.abs(y)))
+
+
+@pytest.mark.parametrize("dims,lens,stretch", CASES)
+@pytest.mark.parametrize("omega,nu", [(1.0, 1), (1.6, 3)])
+def test_rbgs_iterator_bitwise(dims, lens, stretch, omega, nu):
+    g = O.Grid(*dims, l=lens, a=stretch)
+    A = g.matrix()
+    x, b = rand_padded(*dims), rand_padded(*dims)
+    xg = x.copy()
+    rbgs_iterator_poisson_matrix(xg, matrix.MatrixHeptadiagonal(0, A), b, sdm_from(g), nu, omega)
+    O.rbgs_iterator(x, A, b, nu, omega)
+    assert np.array_equal(xg, x)
+
+
+def test_rbgs_solver_matches():
+    dims = (8, 8, 8)
+    g = O.Grid(*dims)
+    A = g.matrix()
+    b = rand_padded(*dims)
+    xg, xo = O.padded(*dims), O.padded(*dims)
+    it, rel = rbgs_solver_poisson_matrix(xg, None, b, sdm_from(g), 1000, 1e-10, 1.0)
+    ito, relo = O.rbgs_solver(xo, A, b, 1000, 1e-10, 1.0)
+    assert it == ito
+    assert np.array_equal(xg, xo)
+    assert abs(rel - relo) <= 1e-14
+
+
+def test_cg_iterator_and_solver():
+    dims = (12, 10, 8)
+    g = O.Grid(*dims)
+    A = g.matrix()
+    x, b = rand_padded(*dims), rand_padded(*dims)
+    xg = x.copy()
+    cg_iterator_poisson_matrix(xg, None, b, sdm_from(g), 6)
+    O.cg_iterator(x, A, b, 6)
+    np.testing.assert_allclose(xg, x, rtol=1e-11, atol=1e-13 * np.abs(x).max())
+    xg, xo = O.padded(*dims), O.padded(*dims)
+    it, rel = cg_solver_poisson_matrix(xg, None, b, sdm_from(g), 400, 1e-9)
+    ito, relo = O.cg_solver(xo, A, b, 400, 1e-9)
+    assert abs(it - ito) <= 1 and rel < 1e-9
+    np.testing.assert_allclose(xg, xo, rtol=1e-8, atol=1e-9 * np.abs(xo).max())
+
+
+VCASES = [  # dims, lens, stretch, nlevel, nu, omega
+    ((32, 32, 32), (1, 1, 1), (1, 1, 1), 3, 2, 1.0),
+    ((64, 64, 64), (1, 1, 1), (1, 1, 1), 4, 2, 1.0),          # config 0 shape
+    ((48, 48, 48), (1, 1, 1), (1, 1, 1), 4, 2, 1.0),          # coarsest 3^3
+    ((32, 32, 32), (1, 1, 2), (1, 1, 1), 3, 2, 1.0),          # hx:hy:hz = 1:1:2
+    ((32, 16, 8), (2, 1, 0.5), (1, 1, 1), 4, 1, 1.2),         # axes stop coarsening at different levels
+    ((32, 32, 32), (1, 1, 1), (1.1, 1.0, 1.05), 3, 2, 1.0),   # stretched
+    ((16, 16, 16), (1, 1, 1), (1, 1, 1), 4, 2, 1.0),          # coarsest 1^3, direct
+]
+
+
+def run_both(dims, lens, stretch, nlevel, nu, omega, ncycle=60, smoother=0, seed=20210501):
+    g = O.Grid(*dims, l=lens, a=stretch)
+    A = g.matrix()
+    b = -O.uniform_field(*dims, seed)
+    mo = O.Multigrid(g, nlevel, ncycle, smoother=smoother)
+    xo, ro = O.padded(*dims), O.padded(*dims)
+    nc, hist, cit, res0 = mo.solve_vcycle(xo, ro, A, b, nu, 1e-10, omega)
+    mg = Multigrid(sdm_from(g), nlevel, ncycle, smoother=smoother)
+    xg, rg = O.padded(*dims), O.padded(*dims)
+    res = mg.solve_vcycle(xg, rg, matrix.MatrixHeptadiagonal(0, A), b, nu, 1e-10, omega)
+    return (xo, ro, nc, hist, cit, res0), (xg, rg, res)
+
+
+@pytest.mark.parametrize("dims,lens,stretch,nlevel,nu,omega", VCASES)
+def test_vcycle_fp64_bitwise(dims, lens, stretch, nlevel, nu, omega):
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both(dims, lens, stretch, nlevel, nu, omega)
+    assert res.cycles == nc and res.converged
+    assert np.max(np.abs(res.history - hist)) <= 1e-12
+    assert abs(res.res0 - res0) <= 1e-13 * res0
+    assert np.array_equal(res.coarse_iters, cit)
+    assert np.array_equal(xg, xo), np.abs(xg - xo).max()
+    assert np.array_equal(rg, ro)
+
+
+def test_vcycle_cg_smoother():
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both((32, 32, 32), (1, 1, 1), (1, 1, 1), 3, 4, 1.0,
+                                                            smoother=1)
+    assert res.cycles == nc and res.converged
+    assert np.max(np.abs(res.history - hist)) <= 1e-12
+    assert np.max(np.abs(xg - xo)) <= 1e-10 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("dims,lens", [((64, 64, 64), (1, 1, 1)), ((32, 32, 32), (1, 1, 2))])
+def test_mixed_precision_vs_fp64_oracle(dims, lens):
+    g = O.Grid(*dims, l=lens)
+    A = g.matrix()
+    b = -O.uniform_field(*dims, 7)
+    nl = int(np.log2(dims[0] // 4))
+    mo = O.Multigrid(g, nl, 80)
+    xo, ro = O.padded(*dims), O.padded(*dims)
+    mo.solve_vcycle(xo, ro, A, b, 2, 1e-10, 1.0)
+    mg = Multigrid(sdm_from(g), nl, 80, precision=1, inner_cycles=4)
+    xg, rg = O.padded(*dims), O.padded(*dims)
+    res = mg.solve_vcycle(xg, rg, None, b, 2, 1e-10, 1.0)
+    assert res.converged and res.rho < 1e-10
+    # the true FP64 residual of the returned solution
+    K = RM.laplacian(g.dxm[1:-1], g.dym[1:-1], g.dzm[1:-1])
+    r = RM.flat(b) - K @ RM.flat(xg)
+    assert np.linalg.norm(r) / np.sqrt(res.res0) < 1.01e-10
+    err = np.abs(xg - xo).max() / np.abs(xo).max()
+    assert err < 1e-8, err
+
+
+def test_ghost_cells_rejected():
+    dims = (8, 8, 8)
+    g = O.Grid(*dims)
+    mg = Multigrid(sdm_from(g), 2, 10)
+    x, r = O.padded(*dims), O.padded(*dims)
+    x[0, 3, 3] = 1.0
+    with pytest.raises(_lib.PepsmgError) as e:
+        mg.solve_vcycle(x, r, None, rand_padded(*dims), 2, 1e-10, 1.0)
+    assert e.value.code == _lib.ERR_GHOST
+
+
+def test_matrix_mismatch_rejected():
+    dims = (8, 8, 8)
+    g = O.Grid(*dims)
+    A = g.matrix()
+    A[2, 2, 2, 3] *= 1.0000001
+    mg = Multigrid(sdm_from(g), 2, 10)
+    x, r = O.padded(*dims), O.padded(*dims)
+    with pytest.raises(_lib.PepsmgError) as e:
+        mg.solve_vcycle(x, r, matrix.MatrixHeptadiagonal(0, A), rand_padded(*dims), 2, 1e-10, 1.0)
+    assert e.value.code == _lib.ERR_MATRIX
+
+
+@pytest.mark.parametrize("nl,aggr,cga", [(1, 0, 0), (2, 5, 0), (2, 0, 2), (9, 0, 0), (5, 2, 0)])
+def test_create_errors(nl, aggr, cga):
+    g = O.Grid(16, 16, 16)
+    with pytest.raises(_lib.PepsmgError) as e:
+        Multigrid(sdm_from(g), nl, 10, aggr, cga)
+    assert e.value.code == _lib.ERR_LEVELS
+
+
+def test_device_api_and_rhs_generator():
+    import torch
+    n = 32
+    g = O.Grid(n, n, n)
+    d_b = torch.zeros((n + 2, n + 2, n + 2), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().pepsmg_fill_rhs_device(n, n, n, C.c_void_p(d_b.data_ptr()), 20210501, 1.0, 0.0,
+                                                 None, None, None, None))
+    torch.cuda.synchronize()
+    assert np.array_equal(d_b.cpu().numpy(), O.uniform_field(n, n, n, 20210501))
+    b = d_b.cpu().numpy()
+    mg = Multigrid(sdm_from(g), 3, 40)
+    d_x = torch.zeros_like(d_b)
+    d_r = torch.zeros_like(d_b)
+    res_d = mg.solve_vcycle_device(d_x.data_ptr(), d_r.data_ptr(), d_b.data_ptr(), 2, 1e-10, 1.0,
+                                   torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    xh, rh = O.padded(n, n, n), O.padded(n, n, n)
+    res_h = mg.solve_vcycle(xh, rh, None, b, 2, 1e-10, 1.0)
+    assert res_d.cycles == res_h.cycles
+    assert np.array_equal(d_x.cpu().numpy(), xh) and np.array_equal(d_r.cpu().numpy(), rh)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_full_size_256_properties(precision):
+    """256^3: convergence and an independent FP64 residual of the returned
+    solution computed with torch on the GPU."""
+    import torch
+    n = 256
+    gdm = geometry.geometry_domain_create(n, n, n, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
+    sdm = geometry.geometry_subdomain_create(gdm)
+    d_b = torch.zeros((n + 2,) * 3, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().pepsmg_fill_rhs_device(n, n, n, C.c_void_p(d_b.data_ptr()), 20210501, -1.0,
+                                                 -3 * np.pi ** 2, _lib.dptr(sdm.xg), _lib.dptr(sdm.yg),
+                                                 _lib.dptr(sdm.zg), None))
+    mg = Multigrid(sdm, 6, 100, precision=precision)
+    d_x = torch.zeros_like(d_b)
+    res = mg.solve_vcycle_device(d_x.data_ptr(), 0, d_b.data_ptr(), 2, 1e-10, 1.0)
+    torch.cuda.synchronize()
+    assert res.converged and res.rho < 1e-10
+    h = 1.0 / n
+    x = d_x
+    c = 1.0 / (h * h)
+    ax = -6 * c * x[1:-1, 1:-1, 1:-1]
+    ax = ax + c * (x[1:-1, 1:-1, :-2] + x[1:-1, 1:-1, 2:] + x[1:-1, :-2, 1:-1] + x[1:-1, 2:, 1:-1] +
+                   x[:-2, 1:-1, 1:-1] + x[2:, 1:-1, 1:-1])
+    # boundary cells: face coefficient 2/h^2 enters the diagonal only
+    ax[:, :, 0] -= c * x[1:-1, 1:-1, 1]
+    ax[:, :, -1] -= c * x[1:-1, 1:-1, -2]
+    ax[:, 0, :] -= c * x[1:-1, 1, 1:-1]
+    ax[:, -1, :] -= c * x[1:-1, -2, 1:-1]
+    ax[0, :, :] -= c * x[1, 1:-1, 1:-1]
+    ax[-1, :, :] -= c * x[-2, 1:-1, 1:-1]
+    r = d_b[1:-1, 1:-1, 1:-1] - ax
+    rho = float(torch.sqrt((r * r).sum() / res.res0))
+    assert rho < 1.05e-10 and abs(rho - res.rho) < 1e-12
