@@ -334,6 +334,12 @@ int pepsmg_multigrid_create(const pepsmg_grid* grid, int32_t nlevel, int32_t ncy
     delete c;
     return hip_fail(e, "hipStreamCreate");
   }
+  {
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 4096)
+      c->lds_max = (size_t)lds - 2048;   // room for the kernel's static LDS
+  }
   rc = c->h64.build(c->host);
   if (rc == PEPSMG_OK) rc = c->alloc_scratch();
   if (rc) {

@@ -1,0 +1,315 @@
+// pepsmg_fused.h -- single-pass red/black iteration (gfx950).
+//
+// One launch performs one full red+black iteration of
+// rbgs_iterator_poisson_matrix (src/rbgs_poisson_matrix.f90:239-271),
+// out of place: x_in is read once, b once, x_out written once (3 n elements,
+// against 6 n for two in-place colour sweeps).  Red cells only read black
+// neighbours of x_in; black cells read the new red values, which each block
+// computes itself for its tile plus a one-cell ring, so blocks never read
+// each other's output and need no inter-block ordering.  Every value is the
+// one the two in-place sweeps produce (same expression and operand order).
+//
+// Blocking: a block owns a TX x TY tile of the x-y plane and marches through
+// a chunk of z planes.  Cells are handled as pairs (i, i+1) with i odd: the
+// two cells of a pair always have opposite colours, so each pair holds
+// exactly one red cell.  LDS stores x_in by pair slot (first / second cell,
+// each a dense array over pairs) and the new red values as one value per
+// pair, so consecutive lanes touch consecutive LDS words.  LDS holds four
+// x_in planes with a two-cell ring and three red planes with a one-cell
+// ring.  Per plane m (two barriers):
+//   red(m) on tile+ring | barrier | black(m-1) on the tile and store of plane
+//   m-1; stage the prefetched plane m+2 into LDS; issue loads of plane m+3
+//   and of the next planes' b values | barrier
+// so global loads are in flight for a whole plane step.  Coefficients of a
+// thread's fixed pairs are kept in registers.
+//
+// Blocks are numbered with the z-chunk fastest, so with 8 (or a multiple of
+// 8) chunks each XCD (blocks are dealt round-robin over the 8 XCDs) owns
+// whole z-slabs and the tile rings are re-read from its own L2.
+//
+// MODE 0: plain.  MODE 1: x_in = 0 (the zeroed coarse x of
+// src/multigrid.f90:1937; nothing is read).
+#pragma once
+#include "pepsmg_kernels.h"
+
+namespace pepsmg {
+
+constexpr int FUSED_THREADS = 256;
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> { using type = double2; };
+template <>
+struct Vec2<float> { using type = float2; };
+
+struct FusedGrid {
+  int gx, gy, nch, kchunk;
+};
+
+// omega * (b - temp) / c0 + (1 - omega) * xold, the reference's SOR update
+// (src/rbgs_poisson_matrix.f90:250).  FP64 divides exactly (bit-identical
+// results); the FP32 inner solver of the mixed-precision path uses the
+// hardware reciprocal-based divide.
+__device__ __forceinline__ double sor_update(double omega, double bv, double temp, double c0, double xold) {
+  return omega * (bv - temp) / c0 + (1.0 - omega) * xold;
+}
+__device__ __forceinline__ float sor_update(float omega, float bv, float temp, float c0, float xold) {
+  return __fdividef(omega * (bv - temp), c0) + (1.0f - omega) * xold;
+}
+
+template <typename T, int TX, int TY, int MODE>
+__global__ void __launch_bounds__(FUSED_THREADS)
+k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G) {
+  constexpr int NT = FUSED_THREADS;
+  constexpr int RP = TX / 2 + 2;            // pairs per row: x = i0-2 .. i0+TX+1
+  constexpr int RY = TY + 4;                // rows: y = j0-2 .. j0+TY+1
+  constexpr int QY = TY + 2;                // red rows: y = j0-1 .. j0+TY
+  constexpr int TP = TX / 2;                // tile pairs per row (pairs 1..TP)
+  constexpr int NXL = (RY * RP + NT - 1) / NT;
+  constexpr int NRED = (QY * RP + NT - 1) / NT;
+  constexpr int NBLK = (TY * TP + NT - 1) / NT;
+  using V2 = typename Vec2<T>::type;
+
+  __shared__ T sa[4][RY][RP];   // x_in, first cell of each pair (i = i0-2+2p, odd)
+  __shared__ T sb[4][RY][RP];   // x_in, second cell (i+1)
+  __shared__ T sr[3][QY][RP];   // new red value of each pair
+
+  const int t = threadIdx.x;
+  const int bid = blockIdx.x;
+  const int bz = bid % G.nch;
+  const int rxy = bid / G.nch;
+  const int i0 = 1 + (rxy % G.gx) * TX;
+  const int j0 = 1 + (rxy / G.gx) * TY;
+  const int k0 = 1 + bz * G.kchunk;
+  const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
+
+  // ---- red pairs of this thread: ring row qy, pair p ----
+  int rq_y[NRED], rq_p[NRED];
+  bool rq_on[NRED], rq_ina[NRED], rq_inb[NRED];
+  T rc_wa[NRED], rc_ea[NRED], rc_wb[NRED], rc_eb[NRED], rc_wy[NRED], rc_ey[NRED];
+#pragma unroll
+  for (int q = 0; q < NRED; q++) {
+    int e = t + q * NT;
+    int qy = e / RP, p = e - qy * RP;
+    rq_y[q] = qy;
+    rq_p[q] = p;
+    int gj = j0 - 1 + qy, ia = i0 - 2 + 2 * p;
+    bool row = e < QY * RP && gj >= 1 && gj <= L.ny;
+    rq_on[q] = row;
+    // first cell is used only from column i0-1 on, second up to i0+TX
+    rq_ina[q] = row && p > 0 && ia >= 1 && ia <= L.nx;
+    rq_inb[q] = row && p < RP - 1 && ia + 1 >= 1 && ia + 1 <= L.nx;
+    rc_wa[q] = rq_ina[q] ? L.cwx[ia] : T(0);
+    rc_ea[q] = rq_ina[q] ? L.cex[ia] : T(0);
+    rc_wb[q] = rq_inb[q] ? L.cwx[ia + 1] : T(0);
+    rc_eb[q] = rq_inb[q] ? L.cex[ia + 1] : T(0);
+    rc_wy[q] = row ? L.cwy[gj] : T(0);
+    rc_ey[q] = row ? L.cey[gj] : T(0);
+  }
+  // ---- black pairs of this thread: tile row ty, pair p (1..TP) ----
+  int bq_y[NBLK], bq_p[NBLK];
+  bool bq_on[NBLK], bq_ina[NBLK], bq_inb[NBLK];
+  T bc_wa[NBLK], bc_ea[NBLK], bc_wb[NBLK], bc_eb[NBLK], bc_wy[NBLK], bc_ey[NBLK];
+#pragma unroll
+  for (int q = 0; q < NBLK; q++) {
+    int e = t + q * NT;
+    int ty = e / TP, p = 1 + (e - ty * TP);
+    bq_y[q] = ty;
+    bq_p[q] = p;
+    int gj = j0 + ty, ia = i0 - 2 + 2 * p;
+    bool row = e < TY * TP && gj <= L.ny;
+    bq_on[q] = row && ia <= L.nx;
+    bq_ina[q] = bq_on[q];
+    bq_inb[q] = bq_on[q] && ia + 1 <= L.nx;
+    bc_wa[q] = bq_ina[q] ? L.cwx[ia] : T(0);
+    bc_ea[q] = bq_ina[q] ? L.cex[ia] : T(0);
+    bc_wb[q] = bq_inb[q] ? L.cwx[ia + 1] : T(0);
+    bc_eb[q] = bq_inb[q] ? L.cex[ia + 1] : T(0);
+    bc_wy[q] = bq_on[q] ? L.cwy[gj] : T(0);
+    bc_ey[q] = bq_on[q] ? L.cey[gj] : T(0);
+  }
+
+  // ---- x_in plane loads ----
+  V2 xpre[NXL];
+  auto issue_x = [&](int m) {
+#pragma unroll
+    for (int q = 0; q < NXL; q++) {
+      int e = t + q * NT;
+      V2 v;
+      v.x = T(0);
+      v.y = T(0);
+      if (MODE != 1 && e < RY * RP && m >= 1 && m <= L.nz) {
+        int ry = e / RP, p = e - ry * RP;
+        int gi = i0 - 2 + 2 * p, gj = j0 - 2 + ry;
+        if (gj >= 1 && gj <= L.ny) {
+          long q0 = lidx(L, gi, gj, m);
+          if (gi >= 1 && gi + 1 <= L.nx) {
+            v = *reinterpret_cast<const V2*>(xi + q0);
+          } else {
+            if (gi >= 1 && gi <= L.nx) v.x = xi[q0];
+            if (gi + 1 >= 1 && gi + 1 <= L.nx) v.y = xi[q0 + 1];
+          }
+        }
+      }
+      xpre[q] = v;
+    }
+  };
+  auto stage_x = [&](int m) {
+#pragma unroll
+    for (int q = 0; q < NXL; q++) {
+      int e = t + q * NT;
+      if (e < RY * RP) {
+        int ry = e / RP, p = e - ry * RP;
+        sa[m & 3][ry][p] = xpre[q].x;
+        sb[m & 3][ry][p] = xpre[q].y;
+      }
+    }
+  };
+
+  // red cell of pair (gj, m): the first cell (i odd) iff gj + m is even
+  T bred[NRED], bblk[NBLK];
+  auto issue_bred = [&](int m) {
+#pragma unroll
+    for (int q = 0; q < NRED; q++) {
+      T v = T(0);
+      if (m >= 1 && m <= L.nz) {
+        int gj = j0 - 1 + rq_y[q];
+        bool first = ((gj + m) & 1) == 0;
+        bool in = first ? rq_ina[q] : rq_inb[q];
+        if (in) v = b[lidx(L, i0 - 2 + 2 * rq_p[q] + (first ? 0 : 1), gj, m)];
+      }
+      bred[q] = v;
+    }
+  };
+  auto issue_bblk = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < NBLK; q++) {
+      T v = T(0);
+      if (k >= 1 && k <= L.nz) {
+        int gj = j0 + bq_y[q];
+        bool bfirst = ((gj + k) & 1) == 1;     // black is the first cell iff gj + k odd
+        bool in = bfirst ? bq_ina[q] : bq_inb[q];
+        if (in) v = b[lidx(L, i0 - 2 + 2 * bq_p[q] + (bfirst ? 0 : 1), gj, k)];
+      }
+      bblk[q] = v;
+    }
+  };
+
+  auto red_plane = [&](int m) {
+    const int s0 = m & 3, sm = (m - 1) & 3, sp = (m + 1) & 3;
+    T(*r)[RP] = sr[(m + 3) % 3];
+    bool zin = (m >= 1 && m <= L.nz);
+    T c5 = T(0), c6 = T(0);
+    if (zin) {
+      c5 = L.cwz[m];
+      c6 = L.cez[m];
+    }
+#pragma unroll
+    for (int q = 0; q < NRED; q++) {
+      int qy = rq_y[q], p = rq_p[q];
+      if (qy >= QY) continue;
+      int ry = qy + 1;                       // x_in row of this ring row
+      int gj = j0 - 1 + qy;
+      bool first = ((gj + m) & 1) == 0;
+      T v = T(0);
+      if (zin && (first ? rq_ina[q] : rq_inb[q])) {
+        T c1 = first ? rc_wa[q] : rc_wb[q], c2 = first ? rc_ea[q] : rc_eb[q];
+        T c3 = rc_wy[q], c4 = rc_ey[q];
+        T c0 = diag(c1, c2, c3, c4, c5, c6);
+        T xw, xe, xs, xn, xd, xu, xc;
+        if (first) {   // i-1 = second cell of pair p-1, i+1 = second cell of pair p
+          xw = sb[s0][ry][p - 1];
+          xe = sb[s0][ry][p];
+          xs = sa[s0][ry - 1][p];
+          xn = sa[s0][ry + 1][p];
+          xd = sa[sm][ry][p];
+          xu = sa[sp][ry][p];
+          xc = sa[s0][ry][p];
+        } else {       // i-1 = first cell of pair p, i+1 = first cell of pair p+1
+          xw = sa[s0][ry][p];
+          xe = sa[s0][ry][p + 1];
+          xs = sb[s0][ry - 1][p];
+          xn = sb[s0][ry + 1][p];
+          xd = sb[sm][ry][p];
+          xu = sb[sp][ry][p];
+          xc = sb[s0][ry][p];
+        }
+        T temp = c1 * xw + c2 * xe + c3 * xs + c4 * xn + c5 * xd + c6 * xu;
+        v = sor_update(omega, bred[q], temp, c0, xc);
+      }
+      r[qy][p] = v;
+    }
+  };
+
+  auto black_plane = [&](int k) {
+    T(*r0)[RP] = sr[k % 3];
+    T(*rm)[RP] = sr[(k + 2) % 3];
+    T(*rp)[RP] = sr[(k + 1) % 3];
+    const int s0 = k & 3;
+    T c5 = L.cwz[k], c6 = L.cez[k];
+#pragma unroll
+    for (int q = 0; q < NBLK; q++) {
+      if (!bq_on[q]) continue;
+      int ty = bq_y[q], p = bq_p[q];
+      int gj = j0 + ty;
+      int ry = ty + 1;                       // red row of tile row ty
+      bool bfirst = ((gj + k) & 1) == 1;
+      bool in = bfirst ? bq_ina[q] : bq_inb[q];
+      T vb = T(0);
+      if (in) {
+        T c1 = bfirst ? bc_wa[q] : bc_wb[q], c2 = bfirst ? bc_ea[q] : bc_eb[q];
+        T c3 = bc_wy[q], c4 = bc_ey[q];
+        T c0 = diag(c1, c2, c3, c4, c5, c6);
+        T xw = bfirst ? r0[ry][p - 1] : r0[ry][p];
+        T xe = bfirst ? r0[ry][p] : r0[ry][p + 1];
+        T temp = c1 * xw + c2 * xe + c3 * r0[ry - 1][p] + c4 * r0[ry + 1][p] + c5 * rm[ry][p] + c6 * rp[ry][p];
+        T xc = bfirst ? sa[s0][ty + 2][p] : sb[s0][ty + 2][p];
+        vb = sor_update(omega, bblk[q], temp, c0, xc);
+      }
+      T vr = r0[ry][p];
+      T lo = bfirst ? vb : vr, hi = bfirst ? vr : vb;
+      int ia = i0 - 2 + 2 * p;
+      long q0 = lidx(L, ia, gj, k);
+      if (bq_inb[q]) {
+        V2 w;
+        w.x = lo;
+        w.y = hi;
+        *reinterpret_cast<V2*>(xo + q0) = w;
+      } else {
+        xo[q0] = lo;
+      }
+    }
+  };
+
+  // prologue: planes k0-2 .. k0 staged, k0+1 in flight
+  if (MODE == 1) {
+    for (int e = t; e < 4 * RY * RP; e += NT) {
+      (&sa[0][0][0])[e] = T(0);
+      (&sb[0][0][0])[e] = T(0);
+    }
+  } else {
+    issue_x(k0 - 2);
+    stage_x(k0 - 2);
+    issue_x(k0 - 1);
+    stage_x(k0 - 1);
+    issue_x(k0);
+    stage_x(k0);
+    issue_x(k0 + 1);
+  }
+  issue_bred(k0 - 1);
+  for (int m = k0 - 1; m <= k1; m++) {
+    __syncthreads();
+    red_plane(m);                            // needs bred(m), x planes m-1..m+1
+    issue_bred(m + 1);
+    __syncthreads();
+    if (m - 1 >= k0) black_plane(m - 1);     // needs bblk(m-1)
+    if (m >= k0) issue_bblk(m);              // b of black plane m, used next step
+    if (MODE != 1) {
+      stage_x(m + 2);
+      issue_x(m + 3);
+    }
+  }
+}
+
+}  // namespace pepsmg

@@ -113,6 +113,7 @@ struct pepsmg_ctx {
   pepsmg::CgBufs<float> cg32;
   pepsmg_options opt;
   hipStream_t stream = nullptr;
+  size_t lds_max = 65536;        // dynamic LDS available to one workgroup
   // scratch
   double* d_partial = nullptr;   // reduction partials
   double* d_scalars = nullptr;   // reduction results / CG scalars

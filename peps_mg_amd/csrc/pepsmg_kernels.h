@@ -237,6 +237,7 @@ __global__ void k_prolong_add(T* __restrict__ xf, const T* __restrict__ xc, Lvl<
 // Deterministic reductions: fixed grid, fixed per-thread order, fixed tree.
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 1024;
+constexpr int PARTIAL_CAP = 1 << 18;   // per-block partial sums of one reduction
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -264,21 +265,23 @@ __global__ void __launch_bounds__(RED_THREADS) k_dot_partial(const T* __restrict
                                                              Lvl<T> L, double* __restrict__ partial,
                                                              T* __restrict__ rout) {
   __shared__ double sh[RED_THREADS / 64];
-  long nrow = (long)L.ny * L.nz;
-  long total = nrow * L.nx;
+  // each wave walks whole rows (j,k); rows are dealt to waves round-robin
+  const int lane = threadIdx.x & 63;
+  const int nw = RED_THREADS / 64;
+  const int nrow = L.ny * L.nz;
   double acc = 0.0;
-  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
-    int i = (int)(c % L.nx) + 1;
-    long row = c / L.nx;
-    int j = (int)(row % L.ny) + 1;
-    int k = (int)(row / L.ny) + 1;
-    if (MODE == 0) {
-      long q = lidx(L, i, j, k);
-      acc += (double)x[q] * (double)y[q];
-    } else {
-      T r = resid_at(x, y, L, i, j, k);
-      if (STORE_R) rout[lidx(L, i, j, k)] = r;
-      acc += (double)r * (double)r;
+  for (int row = blockIdx.x * nw + (threadIdx.x >> 6); row < nrow; row += gridDim.x * nw) {
+    int j = row % L.ny + 1;
+    int k = row / L.ny + 1;
+    for (int i = lane + 1; i <= L.nx; i += 64) {
+      if (MODE == 0) {
+        long q = lidx(L, i, j, k);
+        acc += (double)x[q] * (double)y[q];
+      } else {
+        T r = resid_at(x, y, L, i, j, k);
+        if (STORE_R) rout[lidx(L, i, j, k)] = r;
+        acc += (double)r * (double)r;
+      }
     }
   }
   double s = block_sum(acc, sh);

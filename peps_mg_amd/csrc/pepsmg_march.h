@@ -1,0 +1,330 @@
+// pepsmg_march.h -- residual kernels that march through z planes (gfx950).
+//
+// All of them evaluate r = b - A x exactly as multigrid_residual /
+// mv_mul_poisson_matrix do (src/multigrid.f90:1795-1805,
+// src/poisson_matrix_operator.f90:150-156), with x staged plane by plane in
+// LDS (a 64 x TY tile with a ring of 2 cells in x, so rows load as aligned
+// 16-byte pairs, and 1 cell in y), and the next plane's loads in flight in
+// registers while the current plane is computed.  Each element of x and b
+// is read from HBM once; ring cells are re-read from L2.  Blocks are
+// numbered with the z-chunk fastest (one z-slab per XCD when there are 8 or
+// more chunks).
+//
+//   k_resid_march<T,TY,0>: partial sums of r^2 (convergence check,
+//                          src/multigrid.f90:1970-1971)
+//   k_resid_march<T,TY,1>: same, and stores r
+//   k_resid_march<double,TY,2>: iterative-refinement step: x_in = x + e
+//       (e the FP32 correction), stores x_in, r as float, partial r^2
+//   k_restrict_march<T>:   residual + restriction into the next level's b
+//       (src/multigrid.f90:1932-1933, :1939-1940 with :1460-1494), for
+//       levels that halve every axis
+#pragma once
+#include "pepsmg_fused.h"
+
+namespace pepsmg {
+
+constexpr int MARCH_THREADS = 256;
+
+struct MarchGrid {
+  int gx, gy, nch, kchunk;
+};
+
+// Stages x planes (4 slots) of a 64 x TY tile: x = i0-2 .. i0+65, y = j0-1 .. j0+TY.
+template <typename T, int TY, bool MIXED>
+struct XStager {
+  static constexpr int RX = 68, RY = TY + 2, XP = RX / 2;
+  static constexpr int NXL = (RY * XP + MARCH_THREADS - 1) / MARCH_THREADS;
+  using V2 = typename Vec2<T>::type;
+  V2 pre[NXL];
+
+  __device__ __forceinline__ void issue(const T* __restrict__ x, const float* __restrict__ e, const Lvl<T>& L, int i0,
+                                        int j0, int m) {
+#pragma unroll
+    for (int q = 0; q < NXL; q++) {
+      int idx = threadIdx.x + q * MARCH_THREADS;
+      V2 v;
+      v.x = T(0);
+      v.y = T(0);
+      if (idx < RY * XP && m >= 1 && m <= L.nz) {
+        int ry = idx / XP, px = idx - ry * XP;
+        int gi = i0 - 2 + 2 * px, gj = j0 - 1 + ry;
+        if (gj >= 1 && gj <= L.ny) {
+          long q0 = lidx(L, gi, gj, m);
+          if (gi >= 1 && gi + 1 <= L.nx) {
+            v = *reinterpret_cast<const V2*>(x + q0);
+            if (MIXED) {
+              float2 ev = *reinterpret_cast<const float2*>(e + q0);
+              v.x = v.x + (T)ev.x;
+              v.y = v.y + (T)ev.y;
+            }
+          } else {
+            if (gi >= 1 && gi <= L.nx) v.x = MIXED ? x[q0] + (T)e[q0] : x[q0];
+            if (gi + 1 >= 1 && gi + 1 <= L.nx) v.y = MIXED ? x[q0 + 1] + (T)e[q0 + 1] : x[q0 + 1];
+          }
+        }
+      }
+      pre[q] = v;
+    }
+  }
+  __device__ __forceinline__ void stage(T (*s)[RX]) {
+#pragma unroll
+    for (int q = 0; q < NXL; q++) {
+      int idx = threadIdx.x + q * MARCH_THREADS;
+      if (idx < RY * XP) {
+        int ry = idx / XP, px = idx - ry * XP;
+        s[ry][2 * px] = pre[q].x;
+        s[ry][2 * px + 1] = pre[q].y;
+      }
+    }
+  }
+};
+
+// r at tile cell (lx, ly) of plane slots (s0 = m, sm = m-1, sp = m+1);
+// LDS column of cell lx is lx+2, row ly+1.
+template <typename T, int RX>
+__device__ __forceinline__ T march_resid(T (*s0)[RX], T (*sm)[RX], T (*sp)[RX], int lx, int ly, T c1, T c2, T c3,
+                                         T c4, T c5, T c6, T bv) {
+  T c0 = diag(c1, c2, c3, c4, c5, c6);
+  int cx = lx + 2, cy = ly + 1;
+  T y = c0 * s0[cy][cx] + c1 * s0[cy][cx - 1] + c2 * s0[cy][cx + 1] + c3 * s0[cy - 1][cx] + c4 * s0[cy + 1][cx] +
+        c5 * sm[cy][cx] + c6 * sp[cy][cx];
+  return bv - y;
+}
+
+template <typename T, int TY, int KIND>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __restrict__ b, T* __restrict__ rout,
+              float* __restrict__ r32, T* __restrict__ xnew, Lvl<T> L, MarchGrid G, double* __restrict__ partial) {
+  constexpr bool MIXED = (KIND == 2);
+  using St = XStager<T, TY, MIXED>;
+  constexpr int RX = St::RX, RY = St::RY;
+  constexpr int NP = (TY * 32 + MARCH_THREADS - 1) / MARCH_THREADS;   // cell pairs per thread
+  using V2 = typename Vec2<T>::type;
+  __shared__ T sx[4][RY][RX];
+  __shared__ double sh[MARCH_THREADS / 64];
+
+  const int bid = blockIdx.x;
+  const int bz = bid % G.nch;
+  const int rxy = bid / G.nch;
+  const int i0 = 1 + (rxy % G.gx) * 64;
+  const int j0 = 1 + (rxy / G.gx) * TY;
+  const int k0 = 1 + bz * G.kchunk;
+  const int k1 = min(k0 + G.kchunk, L.nz + 1);
+
+  // per-thread pairs: (ly, lp) -> cells i0+2lp, i0+2lp+1 of row j0+ly
+  T c1a[NP], c2a[NP], c1b[NP], c2b[NP], c3[NP], c4[NP];
+  int ly[NP], lp[NP];
+  bool act[NP];
+#pragma unroll
+  for (int q = 0; q < NP; q++) {
+    int idx = threadIdx.x + q * MARCH_THREADS;
+    ly[q] = idx / 32;
+    lp[q] = idx - ly[q] * 32;
+    int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
+    act[q] = idx < TY * 32 && gj <= L.ny && gi <= L.nx;
+    c1a[q] = act[q] ? L.cwx[gi] : T(0);
+    c2a[q] = act[q] ? L.cex[gi] : T(0);
+    c1b[q] = (act[q] && gi + 1 <= L.nx) ? L.cwx[gi + 1] : T(0);
+    c2b[q] = (act[q] && gi + 1 <= L.nx) ? L.cex[gi + 1] : T(0);
+    c3[q] = act[q] ? L.cwy[gj] : T(0);
+    c4[q] = act[q] ? L.cey[gj] : T(0);
+  }
+
+  St st;
+  V2 bpre[NP];
+  auto issue_b = [&](int m) {
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+      V2 v;
+      v.x = T(0);
+      v.y = T(0);
+      if (act[q] && m >= 1 && m <= L.nz) {
+        int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
+        long q0 = lidx(L, gi, gj, m);
+        if (gi + 1 <= L.nx) v = *reinterpret_cast<const V2*>(b + q0);
+        else v.x = b[q0];
+      }
+      bpre[q] = v;
+    }
+  };
+
+  st.issue(x, e, L, i0, j0, k0 - 1);
+  st.stage(sx[(k0 - 1) & 3]);
+  st.issue(x, e, L, i0, j0, k0);
+  st.stage(sx[k0 & 3]);
+  st.issue(x, e, L, i0, j0, k0 + 1);
+  issue_b(k0);
+  double acc = 0.0;
+  for (int m = k0; m < k1; m++) {
+    st.stage(sx[(m + 1) & 3]);
+    __syncthreads();
+    T(*s0)[RX] = sx[m & 3];
+    T(*sm)[RX] = sx[(m - 1) & 3];
+    T(*sp)[RX] = sx[(m + 1) & 3];
+    T c5 = L.cwz[m], c6 = L.cez[m];
+    V2 bv[NP];
+#pragma unroll
+    for (int q = 0; q < NP; q++) bv[q] = bpre[q];
+    st.issue(x, e, L, i0, j0, m + 2);
+    issue_b(m + 1);
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+      if (!act[q]) continue;
+      int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
+      bool two = gi + 1 <= L.nx;
+      T ra = march_resid<T, RX>(s0, sm, sp, 2 * lp[q], ly[q], c1a[q], c2a[q], c3[q], c4[q], c5, c6, bv[q].x);
+      T rb = two ? march_resid<T, RX>(s0, sm, sp, 2 * lp[q] + 1, ly[q], c1b[q], c2b[q], c3[q], c4[q], c5, c6, bv[q].y)
+                 : T(0);
+      acc += (double)ra * (double)ra;
+      acc += (double)rb * (double)rb;
+      long q0 = lidx(L, gi, gj, m);
+      if (KIND == 1) {
+        if (two) {
+          V2 w;
+          w.x = ra;
+          w.y = rb;
+          *reinterpret_cast<V2*>(rout + q0) = w;
+        } else {
+          rout[q0] = ra;
+        }
+      }
+      if (KIND == 2) {
+        if (two) {
+          V2 w;
+          w.x = s0[ly[q] + 1][2 * lp[q] + 2];
+          w.y = s0[ly[q] + 1][2 * lp[q] + 3];
+          *reinterpret_cast<V2*>(xnew + q0) = w;
+          float2 f;
+          f.x = (float)ra;
+          f.y = (float)rb;
+          *reinterpret_cast<float2*>(r32 + q0) = f;
+        } else {
+          xnew[q0] = s0[ly[q] + 1][2 * lp[q] + 2];
+          r32[q0] = (float)ra;
+        }
+      }
+    }
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[bid] = s;
+}
+
+// Residual + restriction for a level that halves every axis.  Fine tile
+// 64 x 16, coarse tile 32 x 8 (one coarse cell per thread); chunks hold an
+// even number of fine planes.
+template <typename T>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restrict__ b, Lvl<T> L, Lvl<T> Cc,
+                 RestrictW<T> W, MarchGrid G) {
+  constexpr int TY = 16;
+  using St = XStager<T, TY, false>;
+  constexpr int RX = St::RX, RY = St::RY;
+  constexpr int NP = 2;   // pairs per thread: rows ly and ly + 8
+  using V2 = typename Vec2<T>::type;
+  __shared__ T sx[4][RY][RX];
+  __shared__ T sres[2][TY][64];
+
+  const int bid = blockIdx.x;
+  const int bz = bid % G.nch;
+  const int rxy = bid / G.nch;
+  const int i0 = 1 + (rxy % G.gx) * 64;
+  const int j0 = 1 + (rxy / G.gx) * TY;
+  const int k0 = 1 + bz * G.kchunk;                 // odd
+  const int k1 = min(k0 + G.kchunk, L.nz + 1);
+
+  const int t = threadIdx.x;
+  int ly[NP], lp[NP];
+  bool act[NP];
+  T c1a[NP], c2a[NP], c1b[NP], c2b[NP], c3[NP], c4[NP];
+#pragma unroll
+  for (int q = 0; q < NP; q++) {
+    ly[q] = t / 32 + 8 * q;
+    lp[q] = t % 32;
+    int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
+    act[q] = gj <= L.ny && gi <= L.nx;
+    c1a[q] = act[q] ? L.cwx[gi] : T(0);
+    c2a[q] = act[q] ? L.cex[gi] : T(0);
+    c1b[q] = (act[q] && gi + 1 <= L.nx) ? L.cwx[gi + 1] : T(0);
+    c2b[q] = (act[q] && gi + 1 <= L.nx) ? L.cex[gi + 1] : T(0);
+    c3[q] = act[q] ? L.cwy[gj] : T(0);
+    c4[q] = act[q] ? L.cey[gj] : T(0);
+  }
+  // coarse cell of this thread: (ic, jc) = (ic0 + t%32, jc0 + t/32)
+  const int lic = t % 32, ljc = t / 32;
+  const int ic = (i0 + 1) / 2 + lic, jc = (j0 + 1) / 2 + ljc;
+  const bool cact = ic <= Cc.nx && jc <= Cc.ny;
+  const int iz = 2 * ic - 1, ip = 2 * ic, jz = 2 * jc - 1, jp = 2 * jc;
+  T ax1 = T(0), ax2 = T(0), ay1 = T(0), ay2 = T(0);
+  if (cact) {
+    ax1 = W.wx[iz]; ax2 = W.wx[ip]; ay1 = W.wy[jz]; ay2 = W.wy[jp];
+  }
+
+  St st;
+  V2 bpre[NP];
+  auto issue_b = [&](int m) {
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+      V2 v;
+      v.x = T(0);
+      v.y = T(0);
+      if (act[q] && m >= 1 && m <= L.nz) {
+        int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
+        long q0 = lidx(L, gi, gj, m);
+        if (gi + 1 <= L.nx) v = *reinterpret_cast<const V2*>(b + q0);
+        else v.x = b[q0];
+      }
+      bpre[q] = v;
+    }
+  };
+
+  st.issue(x, nullptr, L, i0, j0, k0 - 1);
+  st.stage(sx[(k0 - 1) & 3]);
+  st.issue(x, nullptr, L, i0, j0, k0);
+  st.stage(sx[k0 & 3]);
+  st.issue(x, nullptr, L, i0, j0, k0 + 1);
+  issue_b(k0);
+  for (int m = k0; m < k1; m++) {
+    st.stage(sx[(m + 1) & 3]);
+    __syncthreads();
+    T(*s0)[RX] = sx[m & 3];
+    T(*sm)[RX] = sx[(m - 1) & 3];
+    T(*sp)[RX] = sx[(m + 1) & 3];
+    T c5 = L.cwz[m], c6 = L.cez[m];
+    V2 bv[NP];
+#pragma unroll
+    for (int q = 0; q < NP; q++) bv[q] = bpre[q];
+    st.issue(x, nullptr, L, i0, j0, m + 2);
+    issue_b(m + 1);
+    T(*rr)[64] = sres[(m + 1) & 1];   // fine plane 2kc-1 -> slot 0, 2kc -> slot 1
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+      T ra = T(0), rb = T(0);
+      if (act[q]) {
+        int gi = i0 + 2 * lp[q];
+        ra = march_resid<T, RX>(s0, sm, sp, 2 * lp[q], ly[q], c1a[q], c2a[q], c3[q], c4[q], c5, c6, bv[q].x);
+        if (gi + 1 <= L.nx)
+          rb = march_resid<T, RX>(s0, sm, sp, 2 * lp[q] + 1, ly[q], c1b[q], c2b[q], c3[q], c4[q], c5, c6, bv[q].y);
+      }
+      rr[ly[q]][2 * lp[q]] = ra;
+      rr[ly[q]][2 * lp[q] + 1] = rb;
+    }
+    if ((m & 1) == 0) {
+      __syncthreads();
+      int kc = m / 2;
+      if (cact) {
+        T az1 = W.wz[m - 1], az2 = W.wz[m];
+        T v111 = ax1 * ay1 * az1, v211 = ax2 * ay1 * az1, v121 = ax1 * ay2 * az1, v221 = ax2 * ay2 * az1;
+        T v112 = ax1 * ay1 * az2, v212 = ax2 * ay1 * az2, v122 = ax1 * ay2 * az2, v222 = ax2 * ay2 * az2;
+        T vol_c = v111 + v211 + v121 + v221 + v112 + v212 + v122 + v222;
+        int lxz = 2 * lic, lxp = 2 * lic + 1, lyz = 2 * ljc, lyp = 2 * ljc + 1;
+        // slot 1 = fine plane kp = 2kc, slot 0 = kz = 2kc-1
+        T s = v111 * sres[1][lyp][lxp] + v211 * sres[1][lyp][lxz] + v121 * sres[1][lyz][lxp] +
+              v221 * sres[1][lyz][lxz] + v112 * sres[0][lyp][lxp] + v212 * sres[0][lyp][lxz] +
+              v122 * sres[0][lyz][lxp] + v222 * sres[0][lyz][lxz];
+        bc[lidx(Cc, ic, jc, kc)] = s / vol_c;
+      }
+    }
+  }
+}
+
+}  // namespace pepsmg

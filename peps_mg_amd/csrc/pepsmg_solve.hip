@@ -10,7 +10,10 @@
 #include <vector>
 
 #include "../../include/pepsmg.h"
+#include "pepsmg_coarse.h"
+#include "pepsmg_fused.h"
 #include "pepsmg_internal.h"
+#include "pepsmg_march.h"
 
 namespace pepsmg {
 
@@ -23,6 +26,47 @@ static inline dim3 grid3(int nx, int ny, int nz, int bx, int by) {
 template <typename T>
 struct Ops {
   hipStream_t s;
+  int tile_y = 8;
+
+  // Block decomposition of the fused smoother: 64 x TY tiles, z-chunks of
+  // at least 4 planes, about 2048 blocks, chunk count a multiple of 8 when
+  // there are 8 or more (one z-slab per XCD).
+  // x tile width of the fused smoother: 64 cells in FP64, 128 in FP32 (same
+  // bytes per row)
+  static constexpr int FTX = sizeof(T) == 8 ? 64 : 128;
+
+  static FusedGrid fused_grid(const DevLevel<T>& d, int TX, int TY) {
+    FusedGrid G;
+    G.gx = (d.nx + TX - 1) / TX;
+    G.gy = (d.ny + TY - 1) / TY;
+    long nxy = (long)G.gx * G.gy;
+    long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + 3) / 4));
+    if (nch >= 8) nch = (nch + 7) / 8 * 8;
+    G.kchunk = (int)((d.nz + nch - 1) / nch);
+    G.nch = (d.nz + G.kchunk - 1) / G.kchunk;
+    return G;
+  }
+
+  template <int TY, int MODE>
+  void fused_launch(DevLevel<T>& d, T omega) {
+    FusedGrid G = fused_grid(d, FTX, TY);
+    unsigned nb = (unsigned)(G.gx * G.gy * G.nch);
+    hipLaunchKernelGGL((k_rb_fused<T, FTX, TY, MODE>), dim3(nb), dim3(FUSED_THREADS), 0, s, d.x2, d.x, d.b, d.L(),
+                       omega, G);
+  }
+
+  // One fused red+black iteration x -> x2, then swap.  mode 0 plain, 1 zero
+  // start (x taken as 0).
+  void fused_iter(DevLevel<T>& d, T omega, int mode) {
+    if (tile_y == 16) {
+      if (mode == 1) fused_launch<16, 1>(d, omega);
+      else fused_launch<16, 0>(d, omega);
+    } else {
+      if (mode == 1) fused_launch<8, 1>(d, omega);
+      else fused_launch<8, 0>(d, omega);
+    }
+    std::swap(d.x, d.x2);
+  }
 
   void smooth_colour(DevLevel<T>& d, const T* b, T* x, T omega, int colour, bool zero) {
     int hx = (d.nx + 1) / 2;
@@ -53,12 +97,65 @@ struct Ops {
     hipLaunchKernelGGL((k_apply<T>), grid3(d.nx, d.ny, d.nz, 64, 4), dim3(64, 4, 1), 0, s, y, x, d.L());
   }
 
+  // z-march decomposition: 64 x TY tiles, about 2048 blocks, chunk count a
+  // multiple of 8 when >= 8; even chunks for the restriction kernel.
+  static MarchGrid march_grid(const DevLevel<T>& d, int TY, bool even) {
+    MarchGrid G;
+    G.gx = (d.nx + 63) / 64;
+    G.gy = (d.ny + TY - 1) / TY;
+    long nxy = (long)G.gx * G.gy;
+    long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + 3) / 4));
+    if (nch >= 8) nch = (nch + 7) / 8 * 8;
+    int kc = (int)((d.nz + nch - 1) / nch);
+    if (even && (kc & 1)) kc++;
+    G.kchunk = kc;
+    G.nch = (d.nz + kc - 1) / kc;
+    return G;
+  }
+
   void residual_restrict(DevLevel<T>& f, DevLevel<T>& c, const T* x, const T* b) {
+    if (f.rs[0] == 2 && f.rs[1] == 2 && f.rs[2] == 2) {
+      MarchGrid G = march_grid(f, 16, true);
+      hipLaunchKernelGGL((k_restrict_march<T>), dim3((unsigned)(G.gx * G.gy * G.nch)), dim3(MARCH_THREADS), 0, s,
+                         c.b, x, b, f.L(), c.L(), f.RW(), G);
+      return;
+    }
     hipLaunchKernelGGL((k_restrict<T, true>), grid3(c.nx, c.ny, c.nz, 64, 4), dim3(64, 4, 1), 0, s, c.b, x, b,
                        f.L(), c.L(), f.RW());
   }
 
+  // |b - A x|^2 into out[0] (partial sums per block, then one block); r
+  // stored when non-null.
+  void resid_norm_march(DevLevel<T>& d, const T* x, const T* b, T* r, double* partial, double* out) {
+    MarchGrid G = march_grid(d, 8, false);
+    int nb = G.gx * G.gy * G.nch;
+    if (nb > PARTIAL_CAP) {
+      resid_norm(d, x, b, r, partial, out);
+      return;
+    }
+    if (r)
+      hipLaunchKernelGGL((k_resid_march<T, 8, 1>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, s, x,
+                         (const float*)nullptr, b, r, (float*)nullptr, (T*)nullptr, d.L(), G, partial);
+    else
+      hipLaunchKernelGGL((k_resid_march<T, 8, 0>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, s, x,
+                         (const float*)nullptr, b, (T*)nullptr, (float*)nullptr, (T*)nullptr, d.L(), G, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+  }
+
   void prolong_add(DevLevel<T>& f, DevLevel<T>& c, T* xf) {
+    if (f.ps[0] == 2 && f.ps[1] == 2 && f.ps[2] == 2) {
+      MarchGrid G;
+      G.gx = (f.nx + 63) / 64;
+      G.gy = (f.ny + 15) / 16;
+      long nxy = (long)G.gx * G.gy;
+      long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (c.nz + 1) / 2));
+      if (nch >= 8) nch = (nch + 7) / 8 * 8;
+      G.kchunk = (int)((c.nz + nch - 1) / nch);
+      G.nch = (c.nz + G.kchunk - 1) / G.kchunk;
+      hipLaunchKernelGGL((k_prolong_march<T>), dim3((unsigned)(G.gx * G.gy * G.nch)), dim3(MARCH_THREADS), 0, s, xf,
+                         c.x, f.L(), c.L(), f.PW(), G);
+      return;
+    }
     hipLaunchKernelGGL((k_prolong_add<T>), grid3(f.nx, f.ny, f.nz, 64, 4), dim3(64, 4, 1), 0, s, xf, c.x, f.L(),
                        c.L(), f.PW());
   }
@@ -172,47 +269,13 @@ __global__ void k_f32_to_f64(double* __restrict__ dst, const float* __restrict__
   if (c < n) dst[c] = (double)src[c];
 }
 
-// x_out = x + e; r32 = float(b - A x_out); partial sums of (b - A x_out)^2.
-// Out of place: neighbours of x_out are formed on the fly from x and e.
-__global__ void __launch_bounds__(RED_THREADS) k_refine_update(double* __restrict__ xo, const double* __restrict__ x,
-                                                               const float* __restrict__ e, const double* __restrict__ b,
-                                                               float* __restrict__ r32, Lvl<double> L,
-                                                               double* __restrict__ partial, int first) {
-  __shared__ double sh[RED_THREADS / 64];
-  long total = (long)L.nx * L.ny * L.nz;
-  double acc = 0.0;
-  auto X = [&](int i, int j, int k) -> double {
-    if (i < 1 || i > L.nx || j < 1 || j > L.ny || k < 1 || k > L.nz) return 0.0;
-    long q = lidx(L, i, j, k);
-    return first ? x[q] : x[q] + (double)e[q];
-  };
-  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
-    int i = (int)(c % L.nx) + 1;
-    long row = c / L.nx;
-    int j = (int)(row % L.ny) + 1;
-    int k = (int)(row / L.ny) + 1;
-    double c1 = L.cwx[i], c2 = L.cex[i], c3 = L.cwy[j], c4 = L.cey[j], c5 = L.cwz[k], c6 = L.cez[k];
-    double c0 = diag(c1, c2, c3, c4, c5, c6);
-    double xc = X(i, j, k);
-    double y = c0 * xc + c1 * X(i - 1, j, k) + c2 * X(i + 1, j, k) + c3 * X(i, j - 1, k) + c4 * X(i, j + 1, k) +
-               c5 * X(i, j, k - 1) + c6 * X(i, j, k + 1);
-    long q = lidx(L, i, j, k);
-    double r = b[q] - y;
-    if (!first) xo[q] = xc;
-    r32[q] = (float)r;
-    acc += r * r;
-  }
-  double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
 }  // namespace pepsmg
 
 using namespace pepsmg;
 
 // ---------------------------------------------------------------------------
 int pepsmg_ctx::alloc_scratch() {
-  HIP_TRY(hipMalloc(&d_partial, RED_BLOCKS * sizeof(double)));
+  HIP_TRY(hipMalloc(&d_partial, PARTIAL_CAP * sizeof(double)));
   HIP_TRY(hipMalloc(&d_scalars, 64 * sizeof(double)));
   HIP_TRY(hipMalloc(&d_flag, sizeof(int)));
   citers_cap = std::max(ncycle, 1) + 64;
@@ -244,15 +307,25 @@ void pepsmg_ctx::release() {
 
 namespace pepsmg {
 
-// Smoothing step on level l of hierarchy h (rbgs_iterator or cg_iterator).
+// Smoothing step on level l of hierarchy h (rbgs_iterator or cg_iterator
+// on the level's x and b).  zero_start: x is 0 on entry (coarse levels on
+// the way down, src/multigrid.f90:1937).  prolong_first: first add the
+// prolongation of the next coarser level's x (src/multigrid.f90:1958-1959).
 template <typename T>
-static void smooth(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int l, T* x, const T* b, int nu,
-                   T omega, bool zero_start) {
+static void smooth_level(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int l, int nu, T omega,
+                         bool zero_start, bool prolong_first) {
   DevLevel<T>& d = h.lv[l];
-  if (c->opt.smoother == 1)
-    cg_iterate(op, d, x, b, d.r, cg->p[l], cg->ap[l], nu, zero_start, c->d_partial, c->d_scalars + 8);
-  else
-    op.rbgs(d, x, b, nu, omega, zero_start);
+  bool fused = c->opt.fused_smoother && c->opt.smoother == 0 && nu > 0;
+  if (prolong_first) op.prolong_add(d, h.lv[l + 1], d.x);
+  if (c->opt.smoother == 1) {
+    cg_iterate(op, d, d.x, d.b, d.r, cg->p[l], cg->ap[l], nu, zero_start, c->d_partial, c->d_scalars + 8);
+    return;
+  }
+  if (!fused) {
+    op.rbgs(d, d.x, d.b, nu, omega, zero_start);
+    return;
+  }
+  for (int it = 0; it < nu; it++) op.fused_iter(d, omega, (it == 0 && zero_start) ? 1 : 0);
 }
 
 // Coarsest level: FP64 solve in one workgroup (src/multigrid.f90:1944-1949,
@@ -262,50 +335,118 @@ static void coarse_solve64(pepsmg_ctx* c, DevLevel<double>& d, double omega, dou
                      omega, tol, 1000, (int*)nullptr, slot);
 }
 
-// The down/up legs of one V-cycle below the fine level, for hierarchy h
-// whose level-1 right-hand side has been restricted already
-// (src/multigrid.f90:1936-1962).  In FP32 the coarsest solve runs in FP64.
+// Arguments of the persistent coarse kernel for levels s..L of hierarchy h:
+// compact LDS layout of every level's x and b (16-byte aligned), the
+// coarsest level in FP64.  Returns false if it does not fit in LDS.
 template <typename T>
-static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int nu, T omega, double tol,
-                               int* slot);
-
-template <>
-void vcycle_coarse_part<double>(pepsmg_ctx* c, Ops<double>& op, Hier<double>& h, CgBufs<double>* cg, int nu,
-                                double omega, double tol, int* slot) {
+static bool pc_build(pepsmg_ctx* c, Hier<T>& h, int s, int nu, T omega, double tol, int* slot, PCArgs<T>& A,
+                     size_t& lds) {
   int L = c->nlevel;
-  for (int l = 1; l <= L - 1; l++) {
-    DevLevel<double>& d = h.lv[l];
-    smooth(c, op, h, cg, l, d.x, d.b, nu, omega, true);
-    op.residual_restrict(d, h.lv[l + 1], d.x, d.b);
+  int nlev = L - s + 1;
+  if (s < 1 || nlev > PC_MAXLEV) return false;
+  std::memset(&A, 0, sizeof A);
+  A.nlev = nlev;
+  A.nu = nu;
+  A.omega = omega;
+  A.tol = tol;
+  A.maxit = 1000;
+  A.iters_slot = slot;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = (off + bytes + 15) / 16 * 16;
+    return o;
+  };
+  for (int q = 0; q < nlev; q++) {
+    DevLevel<T>& d = h.lv[s + q];
+    Lvl<T> l = d.L();
+    l.px = d.nx;
+    l.pxy = (long)d.nx * d.ny;
+    A.lv[q] = l;
+    if (q < nlev - 1) {
+      size_t n = (size_t)d.cells();
+      A.xoff[q] = (int)(take(n * sizeof(T)) / sizeof(T));
+      A.boff[q] = (int)(take(n * sizeof(T)) / sizeof(T));
+      A.rw[q] = d.RW();
+      A.pw[q] = d.PW();
+    }
   }
-  coarse_solve64(c, h.lv[L], omega, tol, slot);
-  for (int l = L - 1; l >= 1; l--) {
-    DevLevel<double>& d = h.lv[l];
-    op.prolong_add(d, h.lv[l + 1], d.x);
-    smooth(c, op, h, cg, l, d.x, d.b, nu, omega, false);
-  }
+  DevLevel<double>& cd = c->h64.lv[L];
+  Lvl<double> lc = cd.L();
+  lc.px = cd.nx;
+  lc.pxy = (long)cd.nx * cd.ny;
+  A.lc = lc;
+  size_t nc = (size_t)cd.cells();
+  A.xcoff = (int)(take(nc * 8) / 8);
+  A.bcoff = (int)(take(nc * 8) / 8);
+  A.xctoff = (int)(take(nc * sizeof(T)) / sizeof(T));
+  A.gl = h.lv[s].L();
+  A.gxs = h.lv[s].x;
+  A.gbs = h.lv[s].b;
+  lds = off;
+  return off <= c->lds_max;
 }
 
-template <>
-void vcycle_coarse_part<float>(pepsmg_ctx* c, Ops<float>& op, Hier<float>& h, CgBufs<float>* cg, int nu,
-                               float omega, double tol, int* slot) {
-  int L = c->nlevel;
-  for (int l = 1; l <= L - 1; l++) {
-    DevLevel<float>& d = h.lv[l];
-    smooth(c, op, h, cg, l, d.x, d.b, nu, omega, true);
-    op.residual_restrict(d, h.lv[l + 1], d.x, d.b);
+template <typename T>
+static void pc_launch(pepsmg_ctx* c, const PCArgs<T>& A, size_t lds) {
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void*)k_coarse_vcycle<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)c->lds_max);
+    attr = true;
   }
+  hipLaunchKernelGGL((k_coarse_vcycle<T>), dim3(1), dim3(PC_THREADS), lds, c->stream, A);
+}
+
+// Coarsest level on the per-level path: FP64 solve in one workgroup.  In
+// FP32 the right-hand side is widened to FP64 and the solution narrowed.
+static void coarse_solve_level(pepsmg_ctx* c, Hier<double>& h, double omega, double tol, int* slot) {
+  coarse_solve64(c, h.lv[c->nlevel], omega, tol, slot);
+}
+static void coarse_solve_level(pepsmg_ctx* c, Hier<float>& h, float omega, double tol, int* slot) {
+  int L = c->nlevel;
   DevLevel<float>& cf = h.lv[L];
   DevLevel<double>& cd = c->h64.lv[L];
   long n = (long)cf.ncell;
-  hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, op.s, cd.b, cf.b, n);
+  hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, cd.b, cf.b, n);
   coarse_solve64(c, cd, (double)omega, tol, slot);
-  hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, op.s, cf.x, cd.x, n);
-  for (int l = L - 1; l >= 1; l--) {
-    DevLevel<float>& d = h.lv[l];
-    op.prolong_add(d, h.lv[l + 1], d.x);
-    smooth(c, op, h, cg, l, d.x, d.b, nu, omega, false);
+  hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, cf.x, cd.x, n);
+}
+
+// The down/up legs of one V-cycle below the fine level, for hierarchy h
+// whose level-1 right-hand side has been restricted already
+// (src/multigrid.f90:1936-1962).  Levels from the switch level s down run
+// in the persistent single-workgroup kernel (RBGS smoother only); the
+// coarsest solve is FP64 in every precision.
+template <typename T>
+static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int nu, T omega, double tol,
+                               int* slot) {
+  int L = c->nlevel;
+  int s = L + 1;
+  PCArgs<T> A;
+  size_t lds = 0;
+  if (c->opt.smoother == 0 && c->opt.coarse_switch > 0) {
+    for (int l = 1; l <= L; l++) {
+      if (h.lv[l].cells() <= c->opt.coarse_switch && pc_build(c, h, l, nu, omega, tol, slot, A, lds)) {
+        s = l;
+        break;
+      }
+    }
   }
+  int last = std::min(s, L) - 1;   // last level of the per-level down-leg
+  for (int l = 1; l <= last; l++) {
+    DevLevel<T>& d = h.lv[l];
+    smooth_level(c, op, h, cg, l, nu, omega, true, false);
+    op.residual_restrict(d, h.lv[l + 1], d.x, d.b);
+  }
+  if (s <= L) {
+    A.gxs = h.lv[s].x;   // current buffers
+    A.gbs = h.lv[s].b;
+    pc_launch(c, A, lds);
+  } else {
+    coarse_solve_level(c, h, omega, tol, slot);
+  }
+  for (int l = last; l >= 1; l--) smooth_level(c, op, h, cg, l, nu, omega, false, true);
 }
 
 static void print_cycle(int cyc, double rv, double r0) {
@@ -319,6 +460,7 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
                       std::vector<int>& cit, double* res0_out, int* ncyc_out, double* rho_out) {
   Hier<double>& h = c->h64;
   Ops<double> op{c->stream};
+  op.tile_y = c->opt.tile_y;
   CgBufs<double>* cg = nullptr;
   if (c->opt.smoother == 1) {
     cg = &c->cg64;
@@ -327,7 +469,7 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
   }
   DevLevel<double>& f = h.lv[0];
   double hs[2];
-  op.resid_norm(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 0);      // :1924-1925
+  op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 0);  // :1924-1925
   HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   double res0 = hs[0];
@@ -335,13 +477,12 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
   int ncyc = 0;
   double rho = 1.0;
   for (int cyc = 1; cyc <= c->ncycle; cyc++) {
-    smooth(c, op, h, cg, 0, f.x, f.b, nu, omega, false);                   // :1931
+    smooth_level(c, op, h, cg, 0, nu, omega, false, false);                // :1931
     op.residual_restrict(f, h.lv[1], f.x, f.b);                            // :1932-1933
     int* slot = (cyc - 1 < c->citers_cap) ? c->d_citers + (cyc - 1) : nullptr;
     vcycle_coarse_part<double>(c, op, h, cg, nu, omega, tol, slot);        // :1936-1962
-    op.prolong_add(f, h.lv[1], f.x);                                       // :1965-1966
-    smooth(c, op, h, cg, 0, f.x, f.b, nu, omega, false);                   // :1967
-    op.resid_norm(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 1);   // :1970-1971
+    smooth_level(c, op, h, cg, 0, nu, omega, false, true);                 // :1965-1967
+    op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 1);  // :1970-1971
     HIP_TRY(hipMemcpyAsync(hs, c->d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     rho = std::sqrt(hs[0] / res0);
@@ -368,6 +509,7 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   Hier<double>& h = c->h64;
   Hier<float>& g = c->h32;
   Ops<float> op{c->stream};
+  op.tile_y = c->opt.tile_y;
   CgBufs<float>* cg = nullptr;
   if (c->opt.smoother == 1) {
     cg = &c->cg32;
@@ -376,12 +518,19 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   }
   DevLevel<double>& f = h.lv[0];
   DevLevel<float>& f32 = g.lv[0];
-  int nb = Ops<double>{c->stream}.red_blocks(f);
+  MarchGrid G = Ops<double>::march_grid(f, 8, false);
+  int nb = G.gx * G.gy * G.nch;
+  // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x, x2
+  auto refine_update = [&]() {
+    hipLaunchKernelGGL((k_resid_march<double, 8, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
+                       f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
+    std::swap(f.x, f.x2);
+  };
   double hs[1];
-  // r32 = b - A x; res0
-  hipLaunchKernelGGL(k_refine_update, dim3(nb), dim3(RED_THREADS), 0, c->stream, f.x2, f.x, f32.x, f.b, f32.b, f.L(),
-                     c->d_partial, 1);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
+  // r32 = b - A x and res0, with e = 0
+  hipMemsetAsync(f32.x, 0, f32.ncell * sizeof(float), c->stream);
+  refine_update();
   HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   double res0 = hs[0];
@@ -393,18 +542,13 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   for (int s = 1; s <= c->opt.max_refine && ncyc < c->ncycle; s++) {
     for (int q = 0; q < c->opt.inner_cycles; q++) {
       int* slot = (ncyc < c->citers_cap) ? c->d_citers + ncyc : nullptr;
-      smooth(c, op, g, cg, 0, f32.x, f32.b, nu, om, q == 0);
+      smooth_level(c, op, g, cg, 0, nu, om, q == 0, false);
       op.residual_restrict(f32, g.lv[1], f32.x, f32.b);
       vcycle_coarse_part<float>(c, op, g, cg, nu, om, tol, slot);
-      op.prolong_add(f32, g.lv[1], f32.x);
-      smooth(c, op, g, cg, 0, f32.x, f32.b, nu, om, false);
+      smooth_level(c, op, g, cg, 0, nu, om, false, true);
       ncyc++;
     }
-    // x2 = x + e; r32 = float(b - A x2); |b - A x2|^2
-    hipLaunchKernelGGL(k_refine_update, dim3(nb), dim3(RED_THREADS), 0, c->stream, f.x2, f.x, f32.x, f.b, f32.b,
-                       f.L(), c->d_partial, 0);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
-    std::swap(f.x, f.x2);
+    refine_update();
     HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     rho = std::sqrt(hs[0] / res0);
@@ -574,36 +718,48 @@ int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double
   if (!c || reps < 1) return fail(PEPSMG_ERR_ARG, "invalid argument");
   if (precision == 1 && c->h32.lv.empty()) return fail(PEPSMG_ERR_ARG, "FP32 hierarchy not built");
   float ms = 0;
-  double n = 0, es = precision == 1 ? 4.0 : 8.0;
+  double n = 0, es = precision == 1 ? 4.0 : 8.0, per = 0;
   const char* name = "";
-  // one colour sweep of the two-kernel smoother: reads x (all cells) and b,
-  // writes half of x: 2 n elements per launch
+  bool fused = c->opt.fused_smoother != 0;
+  int ty = c->opt.tile_y == 16 ? 16 : 8;
+  // fused: one red+black iteration reads x and b and writes x once (3 n
+  // elements); colour sweep: reads x and b, writes half of x (2 n)
   auto body = [&](auto& op, auto& d) {
-    for (int q = 0; q < reps; q++) op.smooth_colour(d, d.b, d.x, (decltype(d.x[0] + 0))omega, q & 1, false);
+    for (int q = 0; q < reps; q++) {
+      if (fused)
+        op.fused_iter(d, (decltype(d.x[0] + 0))omega, 0);
+      else
+        op.smooth_colour(d, d.b, d.x, (decltype(d.x[0] + 0))omega, q & 1, false);
+    }
   };
   if (precision == 1) {
     Ops<float> op{c->stream};
+    op.tile_y = ty;
     DevLevel<float>& d = c->h32.lv[0];
-    body(op, d);  // warm
+    body(op, d);
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     body(op, d);
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     n = (double)d.cells();
-    name = "k_rb_colour<float, false>";
+    name = fused ? (ty == 16 ? "k_rb_fused<float, 128, 16, 0>" : "k_rb_fused<float, 128, 8, 0>")
+                 : "k_rb_colour<float, false>";
   } else {
     Ops<double> op{c->stream};
+    op.tile_y = ty;
     DevLevel<double>& d = c->h64.lv[0];
     body(op, d);
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     body(op, d);
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     n = (double)d.cells();
-    name = "k_rb_colour<double, false>";
+    name = fused ? (ty == 16 ? "k_rb_fused<double, 64, 16, 0>" : "k_rb_fused<double, 64, 8, 0>")
+                 : "k_rb_colour<double, false>";
   }
+  per = fused ? 3.0 : 2.0;
   HIP_TRY(hipEventSynchronize(c->ev1));
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   if (avg_ms) *avg_ms = ms / reps;
-  if (bytes) *bytes = 2.0 * n * es;
+  if (bytes) *bytes = per * n * es;
   if (kernel_name) *kernel_name = name;
   return PEPSMG_OK;
 }
@@ -656,7 +812,7 @@ struct OneLevel {
     np = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 2);
     HIP_TRY(hipMalloc(&pad_a, np * sizeof(double)));
     HIP_TRY(hipMalloc(&pad_b, np * sizeof(double)));
-    HIP_TRY(hipMalloc(&partial, RED_BLOCKS * sizeof(double)));
+    HIP_TRY(hipMalloc(&partial, PARTIAL_CAP * sizeof(double)));
     HIP_TRY(hipMalloc(&sc, 16 * sizeof(double)));
     HIP_TRY(hipMalloc(&flag, sizeof(int)));
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -718,7 +874,7 @@ int pepsmg_rbgs_iterator(const pepsmg_grid* grid, double* sol, const double* coe
   if (!rc) rc = o.upload(o.d.b, rhs);
   if (rc) return rc;
   Ops<double> op{o.s};
-  op.rbgs(o.d, o.d.x, o.d.b, maxiteration, omega, false);
+  for (int it = 0; it < maxiteration; it++) op.fused_iter(o.d, omega, 0);
   return o.download(sol, o.d.x, 0);
 }
 

@@ -108,22 +108,27 @@ VCASES = [  # dims, lens, stretch, nlevel, nu, omega
 ]
 
 
-def run_both(dims, lens, stretch, nlevel, nu, omega, ncycle=60, smoother=0, seed=20210501):
+def run_both(dims, lens, stretch, nlevel, nu, omega, ncycle=60, smoother=0, seed=20210501, **opts):
     g = O.Grid(*dims, l=lens, a=stretch)
     A = g.matrix()
     b = -O.uniform_field(*dims, seed)
     mo = O.Multigrid(g, nlevel, ncycle, smoother=smoother)
     xo, ro = O.padded(*dims), O.padded(*dims)
     nc, hist, cit, res0 = mo.solve_vcycle(xo, ro, A, b, nu, 1e-10, omega)
-    mg = Multigrid(sdm_from(g), nlevel, ncycle, smoother=smoother)
+    mg = Multigrid(sdm_from(g), nlevel, ncycle, smoother=smoother, **opts)
     xg, rg = O.padded(*dims), O.padded(*dims)
     res = mg.solve_vcycle(xg, rg, matrix.MatrixHeptadiagonal(0, A), b, nu, 1e-10, omega)
     return (xo, ro, nc, hist, cit, res0), (xg, rg, res)
 
 
+SMOOTHER_PATHS = [dict(fused_smoother=1, tile_y=8), dict(fused_smoother=1, tile_y=16), dict(fused_smoother=0),
+                  dict(fused_smoother=1, coarse_switch=0), dict(fused_smoother=1, coarse_switch=100000)]
+
+
+@pytest.mark.parametrize("opts", SMOOTHER_PATHS, ids=["fused8", "fused16", "colour", "nopersist", "persist-big"])
 @pytest.mark.parametrize("dims,lens,stretch,nlevel,nu,omega", VCASES)
-def test_vcycle_fp64_bitwise(dims, lens, stretch, nlevel, nu, omega):
-    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both(dims, lens, stretch, nlevel, nu, omega)
+def test_vcycle_fp64_bitwise(dims, lens, stretch, nlevel, nu, omega, opts):
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both(dims, lens, stretch, nlevel, nu, omega, **opts)
     assert res.cycles == nc and res.converged
     assert np.max(np.abs(res.history - hist)) <= 1e-12
     assert abs(res.res0 - res0) <= 1e-13 * res0
