@@ -39,7 +39,38 @@ def parse():
     p.add_argument("--inner", type=int, default=0, help="FP32 V-cycles per refinement step (0: default)")
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     p.add_argument("--smoother-reps", type=int, default=20)
+    p.add_argument("--kernels-only", action="store_true",
+                   help="run only the fine-level smoother (FP64 and FP32) and a 1 GiB device copy, for "
+                        "rocprofv3 counter passes; prints one JSON line with the event timings")
     return p.parse_args()
+
+
+def kernels_only(a):
+    """Smoother launches on the 512^3 workload plus a plain copy of known size,
+    the calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    from peps_mg_amd import _lib, geometry
+    from peps_mg_amd.multigrid import Multigrid
+    n = a.n
+    levels = int(round(math.log2(n // 4)))
+    gdm = geometry.geometry_domain_create(n, n, n, 0.0, 0.0, 0.0, 1.0, 1.0, a.lz)
+    sdm = geometry.geometry_subdomain_create(gdm)
+    out = {}
+    for prec in (0, 1):
+        mg = Multigrid(sdm, levels, 1, precision=prec)
+        avg = C.c_double(0)
+        nb = C.c_double(0)
+        kn = C.c_char_p()
+        _lib.check(_lib.lib().pepsmg_bench_smoother(mg.h, prec, a.smoother_reps, 1.0, C.byref(avg), C.byref(nb),
+                                                    C.byref(kn)))
+        out[kn.value.decode()] = {"avg_ms": avg.value, "bytes": nb.value, "launches": 2 * a.smoother_reps}
+        mg.destroy()
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    out["copy_1GiB"] = {"bytes_read": 1 << 30, "bytes_written": 1 << 30}
+    print(json.dumps(out))
 
 
 def log(*a):
@@ -74,6 +105,9 @@ def cpu_baseline(n, lz, fp64_cycles):
 
 def main():
     a = parse()
+    if a.kernels_only:
+        kernels_only(a)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
                                    "and not implemented", "n_gpus": a.gpus}))

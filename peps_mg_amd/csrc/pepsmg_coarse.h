@@ -108,7 +108,7 @@ k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> C
   issue_c(kc0 + 2);
   issue_f(2 * kc0 - 1);
   for (int kc = kc0; kc < kc1; kc++) {
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       int f = 2 * kc - 1 + h;                        // h = 0: odd fine plane, 1: even
@@ -305,6 +305,42 @@ __global__ void __launch_bounds__(PC_THREADS) k_coarse_vcycle(PCArgs<T> A) {
       if (threadIdx.x == 0) {
         x[0] = b[0] / diag(L.cwx[1], L.cex[1], L.cwy[1], L.cey[1], L.cwz[1], L.cez[1]);
         if (A.iters_slot) *A.iters_slot = 0;
+      }
+      __syncthreads();
+    } else if (n <= 64) {
+      // one wavefront, one cell per lane; LDS traffic of a single wave is
+      // ordered, so a wavefront fence replaces the workgroup barriers
+      if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const bool own = lane < n;
+        const int i = lane % L.nx + 1, j = (lane / L.nx) % L.ny + 1, k = lane / (L.nx * L.ny) + 1;
+        const int col = (i + j + k) & 1;     // 1: first sweep (i+j+k odd), 0: second
+        auto wsync = []() {
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        };
+        auto norm = [&]() -> double {
+          double r = own ? resid_at(x, b, L, i, j, k) : 0.0;
+          seq[lane] = r * r;
+          wsync();
+          double s = 0.0;
+          if (lane == 0)
+            for (int c = 0; c < n; c++) s = s + seq[c];
+          s = __shfl(s, 0, 64);
+          wsync();
+          return s;
+        };
+        double r0 = norm();
+        int it;
+        for (it = 0; it < A.maxit; it++) {
+          if (own && col == 1) x[lidx(L, i, j, k)] = sor_at(x, b, L, i, j, k, (double)A.omega, x[lidx(L, i, j, k)]);
+          wsync();
+          if (own && col == 0) x[lidx(L, i, j, k)] = sor_at(x, b, L, i, j, k, (double)A.omega, x[lidx(L, i, j, k)]);
+          wsync();
+          double rn = norm();
+          if (sqrt(rn / r0) <= A.tol) break;
+        }
+        if (lane == 0 && A.iters_slot) *A.iters_slot = it;
       }
       __syncthreads();
     } else {

@@ -14,14 +14,18 @@
 // two cells of a pair always have opposite colours, so each pair holds
 // exactly one red cell.  LDS stores x_in by pair slot (first / second cell,
 // each a dense array over pairs) and the new red values as one value per
-// pair, so consecutive lanes touch consecutive LDS words.  LDS holds four
-// x_in planes with a two-cell ring and three red planes with a one-cell
-// ring.  Per plane m (two barriers):
-//   red(m) on tile+ring | barrier | black(m-1) on the tile and store of plane
-//   m-1; stage the prefetched plane m+2 into LDS; issue loads of plane m+3
-//   and of the next planes' b values | barrier
-// so global loads are in flight for a whole plane step.  Coefficients of a
-// thread's fixed pairs are kept in registers.
+// pair, so consecutive lanes touch consecutive LDS words; each lane picks
+// its slot by address, not by branch.  LDS holds four x_in planes with a
+// two-cell ring and three red planes with a one-cell ring.  Per plane m (two
+// barriers):
+//   red(m) on tile+ring | barrier | black(m-1) on the tile and store of
+//   plane m-1; stage plane m+2 from registers into LDS | barrier
+// Global loads of x (plane m+4) and b (red plane m+2, black plane m+1) are
+// issued two plane steps before use into double-buffered registers (the
+// plane loop is unrolled by two so buffer indices are compile-time).
+// Blocks whose tile plus ring lies inside the domain (most of them) run a
+// specialisation without bounds guards.  Coefficients of a thread's fixed
+// pairs are kept in registers.
 //
 // Blocks are numbered with the z-chunk fastest, so with 8 (or a multiple of
 // 8) chunks each XCD (blocks are dealt round-robin over the 8 XCDs) owns
@@ -30,6 +34,8 @@
 // MODE 0: plain.  MODE 1: x_in = 0 (the zeroed coarse x of
 // src/multigrid.f90:1937; nothing is read).
 #pragma once
+#include <type_traits>
+
 #include "pepsmg_kernels.h"
 
 namespace pepsmg {
@@ -58,35 +64,33 @@ __device__ __forceinline__ float sor_update(float omega, float bv, float temp, f
   return __fdividef(omega * (bv - temp), c0) + (1.0f - omega) * xold;
 }
 
-template <typename T, int TX, int TY, int MODE>
-__global__ void __launch_bounds__(FUSED_THREADS)
-k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G) {
-  constexpr int NT = FUSED_THREADS;
-  constexpr int RP = TX / 2 + 2;            // pairs per row: x = i0-2 .. i0+TX+1
-  constexpr int RY = TY + 4;                // rows: y = j0-2 .. j0+TY+1
-  constexpr int QY = TY + 2;                // red rows: y = j0-1 .. j0+TY
-  constexpr int TP = TX / 2;                // tile pairs per row (pairs 1..TP)
-  constexpr int NXL = (RY * RP + NT - 1) / NT;
-  constexpr int NRED = (QY * RP + NT - 1) / NT;
-  constexpr int NBLK = (TY * TP + NT - 1) / NT;
+template <typename T, int TX, int TY>
+struct FusedShape {
+  static constexpr int NT = FUSED_THREADS;
+  static constexpr int RP = TX / 2 + 2;            // pairs per row: x = i0-2 .. i0+TX+1
+  static constexpr int RY = TY + 4;                // rows: y = j0-2 .. j0+TY+1
+  static constexpr int QY = TY + 2;                // red rows: y = j0-1 .. j0+TY
+  static constexpr int TP = TX / 2;                // tile pairs per row (pairs 1..TP)
+  static constexpr int NXL = (RY * RP + NT - 1) / NT;
+  static constexpr int NRED = (QY * RP + NT - 1) / NT;
+  static constexpr int NBLK = (TY * TP + NT - 1) / NT;
+  static constexpr int PLANE = RY * RP;
+};
+
+// IN: the tile and its two-cell ring lie inside the domain in x and y.
+template <typename T, int TX, int TY, int MODE, bool IN>
+__device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b,
+                                              const Lvl<T>& L, T omega, int i0, int j0, int k0, int k1,
+                                              T* __restrict__ sa, T* __restrict__ sb, T* __restrict__ sr) {
+  using S = FusedShape<T, TX, TY>;
+  constexpr int NT = S::NT, RP = S::RP, RY = S::RY, QY = S::QY, TP = S::TP;
+  constexpr int NXL = S::NXL, NRED = S::NRED, NBLK = S::NBLK, PLANE = S::PLANE;
   using V2 = typename Vec2<T>::type;
-
-  __shared__ T sa[4][RY][RP];   // x_in, first cell of each pair (i = i0-2+2p, odd)
-  __shared__ T sb[4][RY][RP];   // x_in, second cell (i+1)
-  __shared__ T sr[3][QY][RP];   // new red value of each pair
-
   const int t = threadIdx.x;
-  const int bid = blockIdx.x;
-  const int bz = bid % G.nch;
-  const int rxy = bid / G.nch;
-  const int i0 = 1 + (rxy % G.gx) * TX;
-  const int j0 = 1 + (rxy / G.gx) * TY;
-  const int k0 = 1 + bz * G.kchunk;
-  const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
 
   // ---- red pairs of this thread: ring row qy, pair p ----
   int rq_y[NRED], rq_p[NRED];
-  bool rq_on[NRED], rq_ina[NRED], rq_inb[NRED];
+  bool rq_ina[NRED], rq_inb[NRED];
   T rc_wa[NRED], rc_ea[NRED], rc_wb[NRED], rc_eb[NRED], rc_wy[NRED], rc_ey[NRED];
 #pragma unroll
   for (int q = 0; q < NRED; q++) {
@@ -95,11 +99,10 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
     rq_y[q] = qy;
     rq_p[q] = p;
     int gj = j0 - 1 + qy, ia = i0 - 2 + 2 * p;
-    bool row = e < QY * RP && gj >= 1 && gj <= L.ny;
-    rq_on[q] = row;
-    // first cell is used only from column i0-1 on, second up to i0+TX
-    rq_ina[q] = row && p > 0 && ia >= 1 && ia <= L.nx;
-    rq_inb[q] = row && p < RP - 1 && ia + 1 >= 1 && ia + 1 <= L.nx;
+    bool row = e < QY * RP && (IN || (gj >= 1 && gj <= L.ny));
+    // the first cell is used only from column i0-1 on, the second up to i0+TX
+    rq_ina[q] = row && p > 0 && (IN || (ia >= 1 && ia <= L.nx));
+    rq_inb[q] = row && p < RP - 1 && (IN || (ia + 1 >= 1 && ia + 1 <= L.nx));
     rc_wa[q] = rq_ina[q] ? L.cwx[ia] : T(0);
     rc_ea[q] = rq_ina[q] ? L.cex[ia] : T(0);
     rc_wb[q] = rq_inb[q] ? L.cwx[ia + 1] : T(0);
@@ -109,7 +112,7 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
   }
   // ---- black pairs of this thread: tile row ty, pair p (1..TP) ----
   int bq_y[NBLK], bq_p[NBLK];
-  bool bq_on[NBLK], bq_ina[NBLK], bq_inb[NBLK];
+  bool bq_on[NBLK], bq_inb[NBLK];
   T bc_wa[NBLK], bc_ea[NBLK], bc_wb[NBLK], bc_eb[NBLK], bc_wy[NBLK], bc_ey[NBLK];
 #pragma unroll
   for (int q = 0; q < NBLK; q++) {
@@ -118,21 +121,19 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
     bq_y[q] = ty;
     bq_p[q] = p;
     int gj = j0 + ty, ia = i0 - 2 + 2 * p;
-    bool row = e < TY * TP && gj <= L.ny;
-    bq_on[q] = row && ia <= L.nx;
-    bq_ina[q] = bq_on[q];
-    bq_inb[q] = bq_on[q] && ia + 1 <= L.nx;
-    bc_wa[q] = bq_ina[q] ? L.cwx[ia] : T(0);
-    bc_ea[q] = bq_ina[q] ? L.cex[ia] : T(0);
+    bq_on[q] = e < TY * TP && (IN || (gj <= L.ny && ia <= L.nx));
+    bq_inb[q] = bq_on[q] && (IN || ia + 1 <= L.nx);
+    bc_wa[q] = bq_on[q] ? L.cwx[ia] : T(0);
+    bc_ea[q] = bq_on[q] ? L.cex[ia] : T(0);
     bc_wb[q] = bq_inb[q] ? L.cwx[ia + 1] : T(0);
     bc_eb[q] = bq_inb[q] ? L.cex[ia + 1] : T(0);
     bc_wy[q] = bq_on[q] ? L.cwy[gj] : T(0);
     bc_ey[q] = bq_on[q] ? L.cey[gj] : T(0);
   }
 
-  // ---- x_in plane loads ----
-  V2 xpre[NXL];
-  auto issue_x = [&](int m) {
+  // ---- x_in plane loads (pairs), double-buffered by plane parity ----
+  V2 xp[2][NXL];
+  auto issue_x = [&](V2* dst, int m) {
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
       int e = t + q * NT;
@@ -142,7 +143,9 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
       if (MODE != 1 && e < RY * RP && m >= 1 && m <= L.nz) {
         int ry = e / RP, p = e - ry * RP;
         int gi = i0 - 2 + 2 * p, gj = j0 - 2 + ry;
-        if (gj >= 1 && gj <= L.ny) {
+        if (IN) {
+          v = *reinterpret_cast<const V2*>(xi + lidx(L, gi, gj, m));
+        } else if (gj >= 1 && gj <= L.ny) {
           long q0 = lidx(L, gi, gj, m);
           if (gi >= 1 && gi + 1 <= L.nx) {
             v = *reinterpret_cast<const V2*>(xi + q0);
@@ -152,24 +155,23 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
           }
         }
       }
-      xpre[q] = v;
+      dst[q] = v;
     }
   };
-  auto stage_x = [&](int m) {
+  auto stage_x = [&](const V2* src, int m) {
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
       int e = t + q * NT;
       if (e < RY * RP) {
-        int ry = e / RP, p = e - ry * RP;
-        sa[m & 3][ry][p] = xpre[q].x;
-        sb[m & 3][ry][p] = xpre[q].y;
+        sa[(m & 3) * PLANE + e] = src[q].x;
+        sb[(m & 3) * PLANE + e] = src[q].y;
       }
     }
   };
 
   // red cell of pair (gj, m): the first cell (i odd) iff gj + m is even
-  T bred[NRED], bblk[NBLK];
-  auto issue_bred = [&](int m) {
+  T br[2][NRED], bk[2][NBLK];
+  auto issue_bred = [&](T* dst, int m) {
 #pragma unroll
     for (int q = 0; q < NRED; q++) {
       T v = T(0);
@@ -179,26 +181,26 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
         bool in = first ? rq_ina[q] : rq_inb[q];
         if (in) v = b[lidx(L, i0 - 2 + 2 * rq_p[q] + (first ? 0 : 1), gj, m)];
       }
-      bred[q] = v;
+      dst[q] = v;
     }
   };
-  auto issue_bblk = [&](int k) {
+  auto issue_bblk = [&](T* dst, int k) {
 #pragma unroll
     for (int q = 0; q < NBLK; q++) {
       T v = T(0);
       if (k >= 1 && k <= L.nz) {
         int gj = j0 + bq_y[q];
         bool bfirst = ((gj + k) & 1) == 1;     // black is the first cell iff gj + k odd
-        bool in = bfirst ? bq_ina[q] : bq_inb[q];
+        bool in = bfirst ? bq_on[q] : bq_inb[q];
         if (in) v = b[lidx(L, i0 - 2 + 2 * bq_p[q] + (bfirst ? 0 : 1), gj, k)];
       }
-      bblk[q] = v;
+      dst[q] = v;
     }
   };
 
-  auto red_plane = [&](int m) {
+  auto red_plane = [&](const T* bv, int m) {
     const int s0 = m & 3, sm = (m - 1) & 3, sp = (m + 1) & 3;
-    T(*r)[RP] = sr[(m + 3) % 3];
+    T* r = sr + ((m + 3) % 3) * (QY * RP);
     bool zin = (m >= 1 && m <= L.nz);
     T c5 = T(0), c6 = T(0);
     if (zin) {
@@ -211,41 +213,33 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
       if (qy >= QY) continue;
       int ry = qy + 1;                       // x_in row of this ring row
       int gj = j0 - 1 + qy;
-      bool first = ((gj + m) & 1) == 0;
+      int first = ((gj + m) & 1) == 0;       // red cell is the first of its pair
       T v = T(0);
       if (zin && (first ? rq_ina[q] : rq_inb[q])) {
         T c1 = first ? rc_wa[q] : rc_wb[q], c2 = first ? rc_ea[q] : rc_eb[q];
         T c3 = rc_wy[q], c4 = rc_ey[q];
         T c0 = diag(c1, c2, c3, c4, c5, c6);
-        T xw, xe, xs, xn, xd, xu, xc;
-        if (first) {   // i-1 = second cell of pair p-1, i+1 = second cell of pair p
-          xw = sb[s0][ry][p - 1];
-          xe = sb[s0][ry][p];
-          xs = sa[s0][ry - 1][p];
-          xn = sa[s0][ry + 1][p];
-          xd = sa[sm][ry][p];
-          xu = sa[sp][ry][p];
-          xc = sa[s0][ry][p];
-        } else {       // i-1 = first cell of pair p, i+1 = first cell of pair p+1
-          xw = sa[s0][ry][p];
-          xe = sa[s0][ry][p + 1];
-          xs = sb[s0][ry - 1][p];
-          xn = sb[s0][ry + 1][p];
-          xd = sb[sm][ry][p];
-          xu = sb[sp][ry][p];
-          xc = sb[s0][ry][p];
-        }
+        // own slot SS (same column in other rows/planes), other slot OO
+        // (x neighbours: pairs p-1,p if first, else p,p+1)
+        const T* SS = first ? sa : sb;
+        const T* OO = first ? sb : sa;
+        int row = ry * RP + p;
+        int ow = s0 * PLANE + row - first;
+        T xw = OO[ow], xe = OO[ow + 1];
+        T xs = SS[s0 * PLANE + row - RP], xn = SS[s0 * PLANE + row + RP];
+        T xd = SS[sm * PLANE + row], xu = SS[sp * PLANE + row];
+        T xc = SS[s0 * PLANE + row];
         T temp = c1 * xw + c2 * xe + c3 * xs + c4 * xn + c5 * xd + c6 * xu;
-        v = sor_update(omega, bred[q], temp, c0, xc);
+        v = sor_update(omega, bv[q], temp, c0, xc);
       }
-      r[qy][p] = v;
+      r[qy * RP + p] = v;
     }
   };
 
-  auto black_plane = [&](int k) {
-    T(*r0)[RP] = sr[k % 3];
-    T(*rm)[RP] = sr[(k + 2) % 3];
-    T(*rp)[RP] = sr[(k + 1) % 3];
+  auto black_plane = [&](const T* bv, int k) {
+    const T* r0 = sr + (k % 3) * (QY * RP);
+    const T* rm = sr + ((k + 2) % 3) * (QY * RP);
+    const T* rp = sr + ((k + 1) % 3) * (QY * RP);
     const int s0 = k & 3;
     T c5 = L.cwz[k], c6 = L.cez[k];
 #pragma unroll
@@ -254,23 +248,24 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
       int ty = bq_y[q], p = bq_p[q];
       int gj = j0 + ty;
       int ry = ty + 1;                       // red row of tile row ty
-      bool bfirst = ((gj + k) & 1) == 1;
-      bool in = bfirst ? bq_ina[q] : bq_inb[q];
+      int bfirst = ((gj + k) & 1) == 1;      // black cell is the first of its pair
+      bool in = bfirst ? true : bq_inb[q];
+      int row = ry * RP + p;
       T vb = T(0);
       if (in) {
         T c1 = bfirst ? bc_wa[q] : bc_wb[q], c2 = bfirst ? bc_ea[q] : bc_eb[q];
         T c3 = bc_wy[q], c4 = bc_ey[q];
         T c0 = diag(c1, c2, c3, c4, c5, c6);
-        T xw = bfirst ? r0[ry][p - 1] : r0[ry][p];
-        T xe = bfirst ? r0[ry][p] : r0[ry][p + 1];
-        T temp = c1 * xw + c2 * xe + c3 * r0[ry - 1][p] + c4 * r0[ry + 1][p] + c5 * rm[ry][p] + c6 * rp[ry][p];
-        T xc = bfirst ? sa[s0][ty + 2][p] : sb[s0][ty + 2][p];
-        vb = sor_update(omega, bblk[q], temp, c0, xc);
+        int ow = row - bfirst;               // red x neighbours: pairs p-1,p or p,p+1
+        T xw = r0[ow], xe = r0[ow + 1];
+        T temp = c1 * xw + c2 * xe + c3 * r0[row - RP] + c4 * r0[row + RP] + c5 * rm[row] + c6 * rp[row];
+        const T* SS = bfirst ? sa : sb;
+        T xc = SS[s0 * PLANE + (ty + 2) * RP + p];
+        vb = sor_update(omega, bv[q], temp, c0, xc);
       }
-      T vr = r0[ry][p];
+      T vr = r0[row];
       T lo = bfirst ? vb : vr, hi = bfirst ? vr : vb;
-      int ia = i0 - 2 + 2 * p;
-      long q0 = lidx(L, ia, gj, k);
+      long q0 = lidx(L, i0 - 2 + 2 * p, gj, k);
       if (bq_inb[q]) {
         V2 w;
         w.x = lo;
@@ -282,34 +277,66 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
     }
   };
 
-  // prologue: planes k0-2 .. k0 staged, k0+1 in flight
+  // prologue: planes k0-2 .. k0 staged; k0+1 and k0+2 in flight
   if (MODE == 1) {
-    for (int e = t; e < 4 * RY * RP; e += NT) {
-      (&sa[0][0][0])[e] = T(0);
-      (&sb[0][0][0])[e] = T(0);
+    for (int e = t; e < 4 * PLANE; e += NT) {
+      sa[e] = T(0);
+      sb[e] = T(0);
     }
   } else {
-    issue_x(k0 - 2);
-    stage_x(k0 - 2);
-    issue_x(k0 - 1);
-    stage_x(k0 - 1);
-    issue_x(k0);
-    stage_x(k0);
-    issue_x(k0 + 1);
+    issue_x(xp[0], k0 - 2);
+    stage_x(xp[0], k0 - 2);
+    issue_x(xp[0], k0 - 1);
+    stage_x(xp[0], k0 - 1);
+    issue_x(xp[0], k0);
+    stage_x(xp[0], k0);
+    issue_x(xp[1], k0 + 1);
+    issue_x(xp[0], k0 + 2);
   }
-  issue_bred(k0 - 1);
-  for (int m = k0 - 1; m <= k1; m++) {
-    __syncthreads();
-    red_plane(m);                            // needs bred(m), x planes m-1..m+1
-    issue_bred(m + 1);
-    __syncthreads();
-    if (m - 1 >= k0) black_plane(m - 1);     // needs bblk(m-1)
-    if (m >= k0) issue_bblk(m);              // b of black plane m, used next step
+  issue_bred(br[1], k0 - 1);
+  issue_bred(br[0], k0);
+  issue_bblk(bk[0], k0);
+
+  // one plane step; P = (m - k0) & 1 selects the register buffers
+  auto step = [&](int m, auto Pc) {
+    constexpr int P = decltype(Pc)::value;
+    lds_barrier();
+    red_plane(br[P], m);                     // needs b(m), x planes m-1..m+1
+    issue_bred(br[P], m + 2);
+    lds_barrier();
+    if (m - 1 >= k0) black_plane(bk[1 - P], m - 1);
+    if (m + 1 >= k0 + 1) issue_bblk(bk[1 - P], m + 1);
     if (MODE != 1) {
-      stage_x(m + 2);
-      issue_x(m + 3);
+      stage_x(xp[P], m + 2);
+      issue_x(xp[P], m + 4);
     }
+  };
+  for (int m = k0 - 1; m <= k1; m += 2) {
+    step(m, std::integral_constant<int, 1>());
+    if (m + 1 <= k1) step(m + 1, std::integral_constant<int, 0>());
   }
+}
+
+template <typename T, int TX, int TY, int MODE>
+__global__ void __launch_bounds__(FUSED_THREADS)
+k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G) {
+  using S = FusedShape<T, TX, TY>;
+  __shared__ T sa[4 * S::PLANE];   // x_in, first cell of each pair (i = i0-2+2p, odd)
+  __shared__ T sb[4 * S::PLANE];   // x_in, second cell (i+1)
+  __shared__ T sr[3 * S::QY * S::RP];   // new red value of each pair
+
+  const int bid = blockIdx.x;
+  const int bz = bid % G.nch;
+  const int rxy = bid / G.nch;
+  const int i0 = 1 + (rxy % G.gx) * TX;
+  const int j0 = 1 + (rxy / G.gx) * TY;
+  const int k0 = 1 + bz * G.kchunk;
+  const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
+  const bool in = i0 >= 3 && i0 + TX + 1 <= L.nx && j0 >= 3 && j0 + TY + 1 <= L.ny;
+  if (in)
+    rb_fused_body<T, TX, TY, MODE, true>(xo, xi, b, L, omega, i0, j0, k0, k1, sa, sb, sr);
+  else
+    rb_fused_body<T, TX, TY, MODE, false>(xo, xi, b, L, omega, i0, j0, k0, k1, sa, sb, sr);
 }
 
 }  // namespace pepsmg

@@ -43,6 +43,16 @@ struct ProlongW {
   int sx, sy, sz;
 };
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for
+// every outstanding global load (its fence covers global memory), which
+// would drain the register prefetch of the z-march kernels at each plane;
+// their barriers only publish LDS writes.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename T>
 __device__ __forceinline__ long lidx(const Lvl<T>& L, int i, int j, int k) {
   return (long)(k - 1) * L.pxy + (long)(j - 1) * L.px + (i - 1);

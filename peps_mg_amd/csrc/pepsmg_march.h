@@ -157,7 +157,7 @@ k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __r
   double acc = 0.0;
   for (int m = k0; m < k1; m++) {
     st.stage(sx[(m + 1) & 3]);
-    __syncthreads();
+    lds_barrier();
     T(*s0)[RX] = sx[m & 3];
     T(*sm)[RX] = sx[(m - 1) & 3];
     T(*sp)[RX] = sx[(m + 1) & 3];
@@ -285,7 +285,7 @@ k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restric
   issue_b(k0);
   for (int m = k0; m < k1; m++) {
     st.stage(sx[(m + 1) & 3]);
-    __syncthreads();
+    lds_barrier();
     T(*s0)[RX] = sx[m & 3];
     T(*sm)[RX] = sx[(m - 1) & 3];
     T(*sp)[RX] = sx[(m + 1) & 3];
@@ -309,7 +309,7 @@ k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restric
       rr[ly[q]][2 * lp[q] + 1] = rb;
     }
     if ((m & 1) == 0) {
-      __syncthreads();
+      lds_barrier();
       int kc = m / 2;
       if (cact) {
         T az1 = W.wz[m - 1], az2 = W.wz[m];
