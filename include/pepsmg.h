@@ -168,6 +168,12 @@ int pepsmg_fill_rhs_device(int32_t nx, int32_t ny, int32_t nz, double *d_out, ui
                            double scale_rand, double scale_sin, const double *xg,
                            const double *yg, const double *zg, void *stream);
 
+/* The same field into a padded host array (the random part is bit-identical
+ * to the device version; sin comes from the host math library). */
+int pepsmg_fill_rhs_host(int32_t nx, int32_t ny, int32_t nz, double *out, uint64_t seed,
+                         double scale_rand, double scale_sin, const double *xg, const double *yg,
+                         const double *zg);
+
 #ifdef __cplusplus
 }
 #endif

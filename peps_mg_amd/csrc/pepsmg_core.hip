@@ -370,6 +370,30 @@ void pepsmg_multigrid_destroy(pepsmg_ctx* ctx) {
   delete ctx;
 }
 
+int pepsmg_fill_rhs_host(int32_t nx, int32_t ny, int32_t nz, double* out, uint64_t seed, double scale_rand,
+                         double scale_sin, const double* xg, const double* yg, const double* zg) {
+  if (nx < 1 || ny < 1 || nz < 1 || !out) return fail(PEPSMG_ERR_ARG, "invalid argument");
+  if (scale_sin != 0.0 && (!xg || !yg || !zg)) return fail(PEPSMG_ERR_ARG, "cell centres required");
+  const double pi = 3.14159265358979323846;
+  size_t sx = (size_t)nx + 2, sy = (size_t)ny + 2;
+  std::memset(out, 0, sx * sy * ((size_t)nz + 2) * sizeof(double));
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        uint64_t lin = (uint64_t)(i - 1) + (uint64_t)nx * ((uint64_t)(j - 1) + (uint64_t)ny * (uint64_t)(k - 1));
+        uint64_t z = seed ^ lin;
+        z += 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        double v = scale_rand * (2.0 * u - 1.0);
+        if (scale_sin != 0.0) v = v + scale_sin * (std::sin(pi * xg[i]) * std::sin(pi * yg[j]) * std::sin(pi * zg[k]));
+        out[(size_t)i + sx * ((size_t)j + sy * (size_t)k)] = v;
+      }
+  return PEPSMG_OK;
+}
+
 int pepsmg_fill_rhs_device(int32_t nx, int32_t ny, int32_t nz, double* d_out, uint64_t seed, double scale_rand,
                            double scale_sin, const double* xg, const double* yg, const double* zg, void* stream) {
   if (nx < 1 || ny < 1 || nz < 1 || !d_out) return fail(PEPSMG_ERR_ARG, "invalid argument");

@@ -1,0 +1,134 @@
+"""Fortran drop-in (peps_mg_amd/fortran): build, CPU-side behaviour, and the
+driver's solution against the oracle on the GPU."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from peps_mg_amd import fortran_build
+
+EXE = os.path.join(fortran_build.BUILD, "main_peps")
+
+PARA = """&meshes
+nx = {n}
+ny = {n}
+nz = {n}
+/
+&origin
+ox = 0.0d0
+oy = 0.0d0
+oz = 0.0d0
+/
+&length
+lx = 1.0d0
+ly = 1.0d0
+lz = {lz}
+/
+&mesh_stretch
+ax = 1.0d0
+ay = 1.0d0
+az = 1.0d0
+/
+&procs
+npx = {npx}
+npy = 1
+npz = 1
+/
+&control
+maxiteration = 2
+tolerance = 1.0d-10
+number_of_vcycles = 60
+number_of_levels = {nl}
+aggregation_method = {aggr}
+CGA_level = 0
+CGPSA_level = 0, 0, 0
+omega_sor = 1.0d0
+/
+&coefficients
+alpha_x = 1.0d0
+alpha_y = 1.0d0
+alpha_z = 1.0d0
+/
+"""
+
+
+def ensure_built():
+    if not os.path.exists(EXE):
+        fortran_build.build()
+    return EXE
+
+
+def write_input(d, n=16, lz="1.0d0", npx=1, nl=2, aggr=0, gpu=None):
+    txt = PARA.format(n=n, lz=lz, npx=npx, nl=nl, aggr=aggr)
+    if gpu:
+        txt += "&gpu\n" + "\n".join(f"{k} = {v}" for k, v in gpu.items()) + "\n/\n"
+    with open(os.path.join(d, "PARA_INPUT.inp"), "w") as f:
+        f.write(txt)
+
+
+def test_fortran_dropin_builds_with_reference_module_names():
+    ensure_built()
+    for m in ["mpi_topology", "geometry", "matrix", "poisson_matrix_operator", "rbgs_poisson_matrix",
+              "cg_poisson_matrix", "multigrid"]:
+        assert os.path.exists(os.path.join(fortran_build.BUILD, m + ".mod")), m
+    syms = subprocess.run(["nm", os.path.join(fortran_build.BUILD, "libpepsmg_f.a")], capture_output=True,
+                          text=True, check=True).stdout
+    for p in ["multigrid_create", "multigrid_solve_vcycle", "multigrid_destroy", "rbgs_iterator_poisson_matrix",
+              "rbgs_solver_poisson_matrix", "cg_iterator_poisson_matrix", "cg_solver_poisson_matrix",
+              "mv_mul_poisson_matrix", "vv_dot_3d_matrix", "matrix_heptadiagonal_create", "geometry_domain_create"]:
+        assert re.search(p, syms, re.I), p
+
+
+def test_driver_rejects_several_processes(tmp_path):
+    """npx > 1 stops before any GPU call (one GPU owns the grid)."""
+    exe = ensure_built()
+    write_input(tmp_path, npx=4)
+    p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert p.returncode != 0
+    assert "one process" in p.stdout
+
+
+@pytest.mark.gpu
+def test_driver_matches_oracle(tmp_path):
+    from oracle import oracle as O
+    from peps_mg_amd import _lib
+    exe = ensure_built()
+    n, nl = 32, 3
+    write_input(tmp_path, n=n, nl=nl, gpu=dict(rhs_mode=1, seed=7, output_file="'x.bin'"))
+    p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    cyc = [float(m.group(1)) for m in re.finditer(r"\[MG\] cycle =\s+\d+, Error:\s+\S+\s+\S+\s+(\S+)", p.stdout)]
+    x = np.fromfile(tmp_path / "x.bin", dtype=np.float64).reshape(n + 2, n + 2, n + 2)
+    g = O.Grid(n, n, n)
+    b = O.padded(n, n, n)
+    _lib.check(_lib.lib().pepsmg_fill_rhs_host(n, n, n, _lib.dptr(b), 7, -1.0, -3 * np.pi ** 2, _lib.dptr(g.xg),
+                                               _lib.dptr(g.yg), _lib.dptr(g.zg)))
+    mo = O.Multigrid(g, nl, 60)
+    xo, ro = O.padded(n, n, n), O.padded(n, n, n)
+    nc, hist, cit, res0 = mo.solve_vcycle(xo, ro, g.matrix(), b, 2, 1e-10, 1.0)
+    assert len(cyc) == nc
+    np.testing.assert_allclose(cyc, hist, rtol=1e-6)       # printed with 8 significant digits
+    assert np.array_equal(x, xo)
+
+
+@pytest.mark.gpu
+def test_driver_cosine_problem(tmp_path):
+    """The reference's Problem 1 (cosine RHS) converges with O(h^2) RMS error."""
+    exe = ensure_built()
+    write_input(tmp_path, n=32, nl=3)
+    p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    rms = float(re.search(r"RMS =\s+(\S+)", p.stdout).group(1))
+    assert rms < 1e-5, p.stdout
+    assert "converged =   1" in p.stdout
+
+
+@pytest.mark.gpu
+def test_driver_level_error_stops(tmp_path):
+    exe = ensure_built()
+    write_input(tmp_path, n=16, nl=9)
+    p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode != 0
+    assert "multigrid_create failed" in p.stdout
