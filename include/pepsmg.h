@@ -59,7 +59,7 @@ typedef struct pepsmg_options {
   int32_t precision;       /* 0: FP64; 1: FP32 V-cycles inside FP64 iterative refinement */
   int32_t coarse_switch;   /* levels with at most this many cells run inside one
                               persistent workgroup (0 = off); default 4096 (16^3) */
-  int32_t inner_cycles;    /* precision 1: FP32 V-cycles per refinement step (default 4) */
+  int32_t inner_cycles;    /* precision 1: FP32 V-cycles per refinement step (default 3) */
   int32_t max_refine;      /* precision 1: cap on refinement steps (default 30) */
   int32_t fused_smoother;  /* 1 (default): single-pass out-of-place red+black kernel; 0: two colour sweeps */
   int32_t tile_y;          /* fused smoother tile height in cells (0 = autotuned default) */

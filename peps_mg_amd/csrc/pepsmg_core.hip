@@ -307,7 +307,7 @@ void pepsmg_default_options(pepsmg_options* o) {
   o->smoother = 0;
   o->precision = 0;
   o->coarse_switch = 4096;
-  o->inner_cycles = 4;
+  o->inner_cycles = 3;
   o->max_refine = 30;
   o->fused_smoother = 1;
   o->tile_y = 0;
