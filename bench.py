@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--inner", type=int, default=0, help="FP32 V-cycles per refinement step (0: default)")
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     p.add_argument("--smoother-reps", type=int, default=20)
+    p.add_argument("--adjoint", action="store_true", help="run the adjoint inverse-source workload instead")
+    p.add_argument("--adjoint-n", type=int, default=128)
     p.add_argument("--kernels-only", action="store_true",
                    help="run only the fine-level smoother (FP64 and FP32) and a 1 GiB device copy, for "
                         "rocprofv3 counter passes; prints one JSON line with the event timings")
@@ -77,6 +79,87 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def adjoint_workload(a):
+    """Inverse source problem on a 128^3 grid: recover f from u_obs, where
+    -lap(u) = f (A u = -f), u_obs = u(f*) + N(0, 1e-3) and f* is a sum of six
+    Gaussians.  Loss 0.5 |u(f) - u_obs|^2 + 1e-4 |grad f|^2 (forward
+    differences / h); gradients through the adjoint multigrid solve.  Reports
+    the wall time of 50 L-BFGS iterations (torch.optim.LBFGS) in FP64 and
+    mixed precision, and a central-difference check (eps 1e-6, 32 random
+    unit directions) of the gradient."""
+    import numpy as np
+    import torch
+    from peps_mg_amd.adjoint import PoissonSolver, poisson_solve
+    n = a.adjoint_n
+    levels = int(round(math.log2(n // 4)))
+    rng = np.random.default_rng(20210501)
+    h = 1.0 / n
+    c = (np.arange(n) + 0.5) * h
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    fstar = np.zeros((n, n, n))
+    for _ in range(6):
+        cx, cy, cz = rng.uniform(0.2, 0.8, 3)
+        w = rng.uniform(0.03, 0.1)
+        amp = rng.uniform(-1, 1)
+        fstar += amp * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / (2 * w * w))
+    dev = torch.device("cuda:0")
+    out = {}
+    for prec in (0, 1):
+        S = PoissonSolver.uniform(n, levels, tolerance=1e-10, precision=prec)
+        ft = torch.tensor(fstar, device=dev)
+        with torch.no_grad():
+            uobs = S.solve(-ft) + torch.tensor(rng.normal(scale=1e-3, size=(n, n, n)), device=dev)
+
+        def loss_fn(f):
+            u = poisson_solve(-f, S)
+            reg = sum(((torch.diff(f, dim=d) / h) ** 2).sum() for d in range(3))
+            return 0.5 * ((u - uobs) ** 2).sum() + 1e-4 * reg
+
+        # central-difference check of the gradient at a random point
+        f0 = torch.tensor(rng.uniform(-1, 1, (n, n, n)), device=dev, requires_grad=True)
+        loss_fn(f0).backward()
+        g = f0.grad.detach()
+        eps, worst = 1e-6, 0.0
+        with torch.no_grad():
+            for _ in range(32):
+                d = torch.tensor(rng.normal(size=(n, n, n)), device=dev)
+                d /= d.norm()
+                fd = (loss_fn(f0 + eps * d) - loss_fn(f0 - eps * d)).item() / (2 * eps)
+                worst = max(worst, abs(fd - (g * d).sum().item()) / g.norm().item())
+        f = torch.zeros((n, n, n), dtype=torch.float64, device=dev, requires_grad=True)
+        opt = torch.optim.LBFGS([f], lr=1.0, max_iter=50, max_eval=10 ** 6, history_size=10,
+                                tolerance_grad=0.0, tolerance_change=0.0, line_search_fn="strong_wolfe")
+        evals = [0]
+
+        def closure():
+            opt.zero_grad()
+            L = loss_fn(f)
+            L.backward()
+            evals[0] += 1
+            return L
+
+        with torch.no_grad():
+            l0 = loss_fn(f).item()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        opt.step(closure)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        with torch.no_grad():
+            l1 = loss_fn(f).item()
+            rel = (torch.linalg.vector_norm(f - ft) / torch.linalg.vector_norm(ft)).item()
+        tag = "fp64" if prec == 0 else "mixed"
+        log(f"[bench] adjoint {tag}: 50 L-BFGS iterations in {dt:.3f} s ({evals[0]} evaluations), "
+            f"loss {l0:.4e} -> {l1:.4e}, FD worst {worst:.2e}")
+        out[tag] = {"lbfgs50_s": round(dt, 4), "evaluations": evals[0], "loss0": l0, "loss50": l1,
+                    "rel_error_vs_fstar": rel, "fd_worst_rel": worst}
+        S.destroy()
+    print(json.dumps({"metric": "adjoint inverse source, 50 L-BFGS iterations", "value": out["mixed"]["lbfgs50_s"],
+                      "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+                      "config": {"workload": f"inverse source, {n}^3, 6 Gaussians, noise 1e-3, reg 1e-4 |grad f|^2"},
+                      "fp64": out["fp64"], "mixed": out["mixed"]}))
+
+
 def cpu_baseline(n, lz, fp64_cycles):
     """Oracle (C restatement, OpenMP) on 2 V-cycles of the same workload."""
     import numpy as np
@@ -105,13 +188,16 @@ def cpu_baseline(n, lz, fp64_cycles):
 
 def main():
     a = parse()
-    if a.kernels_only:
-        kernels_only(a)
-        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
                                    "and not implemented", "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.kernels_only:
+        kernels_only(a)
+        return
+    if a.adjoint:
+        adjoint_workload(a)
+        return
     import numpy as np
     import torch
     from peps_mg_amd import _lib, geometry
