@@ -16,8 +16,9 @@ namespace pepsmg {
 // (f odd: dxp(f), dxp(f+1), c-1; f even: dxn(f), dxn(f-1), c+1); the eight
 // terms are summed in the reference order.
 template <typename T>
-__global__ void __launch_bounds__(MARCH_THREADS)
-k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> Cc, ProlongW<T> W, MarchGrid G) {
+__device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf, const T* __restrict__ xc,
+                                                    const Lvl<T>& F, const Lvl<T>& Cc, const ProlongW<T>& W,
+                                                    const MarchGrid& G) {
   constexpr int CX = 34, CY = 10;                    // coarse tile 32 x 8 + ring 1
   constexpr int NCL = (CX * CY + MARCH_THREADS - 1) / MARCH_THREADS;
   constexpr int NP = 2;                              // fine pairs per thread
@@ -25,7 +26,6 @@ k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> C
   __shared__ T sc[4][CY][CX];
 
   const int t = threadIdx.x;
-  const int bid = blockIdx.x;
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
   const int i0 = 1 + (rxy % G.gx) * 64;              // fine tile origin
@@ -117,8 +117,8 @@ k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> C
 #pragma unroll
       for (int q = 0; q < NP; q++) xv[q] = xpre[q];
       issue_f(f + 1);
-      T wsz = (h == 0) ? W.pz_[f] : W.nz_[f];
-      T wnz = (h == 0) ? W.pz_[f + 1] : W.nz_[f - 1];
+      T wsz = (h == 0) ? ldc(W.pz_, f) : ldc(W.nz_, f);
+      T wnz = (h == 0) ? ldc(W.pz_, f + 1) : ldc(W.nz_, f - 1);
       T(*P)[CX] = sc[kc & 3];
       T(*N)[CX] = sc[(h == 0 ? kc - 1 : kc + 1) & 3];
 #pragma unroll
@@ -151,6 +151,12 @@ k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> C
     stage_c(kc + 2);
     issue_c(kc + 3);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> Cc, ProlongW<T> W, MarchGrid G) {
+  prolong_march_block<T>(blockIdx.x, xf, xc, F, Cc, W, G);
 }
 
 // ---------------------------------------------------------------------------
@@ -262,7 +268,7 @@ __device__ void pc_prolong_add(T* xf, const T* xc, const Lvl<T>& F, const Lvl<T>
 }
 
 template <typename T>
-__global__ void __launch_bounds__(PC_THREADS) k_coarse_vcycle(PCArgs<T> A) {
+__device__ __forceinline__ void coarse_vcycle_block(const PCArgs<T>& A) {
   extern __shared__ __align__(16) unsigned char pc_smem[];
   T* sm = reinterpret_cast<T*>(pc_smem);
   double* smd = reinterpret_cast<double*>(pc_smem);
@@ -382,6 +388,11 @@ __global__ void __launch_bounds__(PC_THREADS) k_coarse_vcycle(PCArgs<T> A) {
       A.gxs[lidx(A.gl, i, j, k)] = (top == 0) ? (T)smd[A.xcoff + c] : sm[A.xoff[0] + c];
     }
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(PC_THREADS) k_coarse_vcycle(PCArgs<T> A) {
+  coarse_vcycle_block<T>(A);
 }
 
 }  // namespace pepsmg

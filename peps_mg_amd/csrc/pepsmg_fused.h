@@ -78,7 +78,7 @@ struct FusedShape {
 };
 
 // IN: the tile and its two-cell ring lie inside the domain in x and y.
-template <typename T, int TX, int TY, int MODE, bool IN>
+template <typename T, int TX, int TY, int MODE, int PF, bool IN>
 __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b,
                                               const Lvl<T>& L, T omega, int i0, int j0, int k0, int k1,
                                               T* __restrict__ sa, T* __restrict__ sb, T* __restrict__ sr) {
@@ -131,70 +131,70 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     bc_ey[q] = bq_on[q] ? L.cey[gj] : T(0);
   }
 
-  // ---- x_in plane loads (pairs), double-buffered by plane parity ----
-  V2 xp[2][NXL];
+  // ---- x_in plane loads (pairs); PF buffers (indexed by plane parity when PF = 2) ----
+  V2 xp[PF][NXL];
+  // Loads are issued unconditionally from clamped in-bounds addresses; the
+  // values that lie outside the domain are zeroed only when they are used
+  // (stage_x) or never read (b of cells outside the domain).  With no branch
+  // or select right after a load the compiler waits on exactly the loads a
+  // plane needs (vmcnt(n) with n > 0) instead of draining the prefetches.
   auto issue_x = [&](V2* dst, int m) {
+    const int mc = min(max(m, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
       int e = t + q * NT;
-      V2 v;
-      v.x = T(0);
-      v.y = T(0);
-      if (MODE != 1 && e < RY * RP && m >= 1 && m <= L.nz) {
-        int ry = e / RP, p = e - ry * RP;
-        int gi = i0 - 2 + 2 * p, gj = j0 - 2 + ry;
-        if (IN) {
-          v = *reinterpret_cast<const V2*>(xi + lidx(L, gi, gj, m));
-        } else if (gj >= 1 && gj <= L.ny) {
-          long q0 = lidx(L, gi, gj, m);
-          if (gi >= 1 && gi + 1 <= L.nx) {
-            v = *reinterpret_cast<const V2*>(xi + q0);
-          } else {
-            if (gi >= 1 && gi <= L.nx) v.x = xi[q0];
-            if (gi + 1 >= 1 && gi + 1 <= L.nx) v.y = xi[q0 + 1];
-          }
-        }
-      }
-      dst[q] = v;
+      int ec = min(e, RY * RP - 1);
+      int ry = ec / RP, p = ec - ry * RP;
+      int gi = i0 - 2 + 2 * p, gj = j0 - 2 + ry;    // gi is odd
+      // an odd gi <= nx keeps the 2-element load inside the padded row
+      int gic = IN ? gi : ((gi >= 1 && gi <= L.nx) ? gi : 1);
+      int gjc = IN ? gj : min(max(gj, 1), L.ny);
+      dst[q] = *reinterpret_cast<const V2*>(xi + lidx(L, gic, gjc, mc));
     }
   };
   auto stage_x = [&](const V2* src, int m) {
+    const bool vz = m >= 1 && m <= L.nz;
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
       int e = t + q * NT;
       if (e < RY * RP) {
-        sa[(m & 3) * PLANE + e] = src[q].x;
-        sb[(m & 3) * PLANE + e] = src[q].y;
+        int ry = e / RP, p = e - ry * RP;
+        int gi = i0 - 2 + 2 * p, gj = j0 - 2 + ry;
+        bool vrow = vz && (IN || (gj >= 1 && gj <= L.ny));
+        bool va = vrow && (IN || (gi >= 1 && gi <= L.nx));
+        bool vb = vrow && (IN || (gi + 1 >= 1 && gi + 1 <= L.nx));
+        sa[(m & 3) * PLANE + e] = va ? src[q].x : T(0);
+        sb[(m & 3) * PLANE + e] = vb ? src[q].y : T(0);
       }
     }
   };
 
   // red cell of pair (gj, m): the first cell (i odd) iff gj + m is even
-  T br[2][NRED], bk[2][NBLK];
+  T br[PF][NRED], bk[PF][NBLK];
+  // b of the red (black) cell of each pair; cells outside the domain load a
+  // clamped address and their value is never used
   auto issue_bred = [&](T* dst, int m) {
+    const int mc = min(max(m, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NRED; q++) {
-      T v = T(0);
-      if (m >= 1 && m <= L.nz) {
-        int gj = j0 - 1 + rq_y[q];
-        bool first = ((gj + m) & 1) == 0;
-        bool in = first ? rq_ina[q] : rq_inb[q];
-        if (in) v = b[lidx(L, i0 - 2 + 2 * rq_p[q] + (first ? 0 : 1), gj, m)];
-      }
-      dst[q] = v;
+      int gj = j0 - 1 + rq_y[q];
+      bool first = ((gj + m) & 1) == 0;
+      bool in = first ? rq_ina[q] : rq_inb[q];
+      int gi = i0 - 2 + 2 * rq_p[q] + (first ? 0 : 1);
+      int gic = in ? gi : 1, gjc = in ? gj : 1;
+      dst[q] = b[lidx(L, gic, gjc, mc)];
     }
   };
   auto issue_bblk = [&](T* dst, int k) {
+    const int kc = min(max(k, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NBLK; q++) {
-      T v = T(0);
-      if (k >= 1 && k <= L.nz) {
-        int gj = j0 + bq_y[q];
-        bool bfirst = ((gj + k) & 1) == 1;     // black is the first cell iff gj + k odd
-        bool in = bfirst ? bq_on[q] : bq_inb[q];
-        if (in) v = b[lidx(L, i0 - 2 + 2 * bq_p[q] + (bfirst ? 0 : 1), gj, k)];
-      }
-      dst[q] = v;
+      int gj = j0 + bq_y[q];
+      bool bfirst = ((gj + k) & 1) == 1;     // black is the first cell iff gj + k odd
+      bool in = bfirst ? bq_on[q] : bq_inb[q];
+      int gi = i0 - 2 + 2 * bq_p[q] + (bfirst ? 0 : 1);
+      int gic = in ? gi : 1, gjc = in ? gj : 1;
+      dst[q] = b[lidx(L, gic, gjc, kc)];
     }
   };
 
@@ -204,8 +204,8 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     bool zin = (m >= 1 && m <= L.nz);
     T c5 = T(0), c6 = T(0);
     if (zin) {
-      c5 = L.cwz[m];
-      c6 = L.cez[m];
+      c5 = ldc(L.cwz, m);
+      c6 = ldc(L.cez, m);
     }
 #pragma unroll
     for (int q = 0; q < NRED; q++) {
@@ -241,7 +241,7 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     const T* rm = sr + ((k + 2) % 3) * (QY * RP);
     const T* rp = sr + ((k + 1) % 3) * (QY * RP);
     const int s0 = k & 3;
-    T c5 = L.cwz[k], c6 = L.cez[k];
+    T c5 = ldc(L.cwz, k), c6 = ldc(L.cez, k);
 #pragma unroll
     for (int q = 0; q < NBLK; q++) {
       if (!bq_on[q]) continue;
@@ -277,7 +277,7 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     }
   };
 
-  // prologue: planes k0-2 .. k0 staged; k0+1 and k0+2 in flight
+  // prologue: planes k0-2 .. k0 staged; k0+1 .. k0+PF in flight
   if (MODE == 1) {
     for (int e = t; e < 4 * PLANE; e += NT) {
       sa[e] = T(0);
@@ -290,25 +290,36 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     stage_x(xp[0], k0 - 1);
     issue_x(xp[0], k0);
     stage_x(xp[0], k0);
-    issue_x(xp[1], k0 + 1);
-    issue_x(xp[0], k0 + 2);
+    if (PF == 2) {
+      issue_x(xp[1 % PF], k0 + 1);
+      issue_x(xp[0], k0 + 2);
+    } else {
+      issue_x(xp[0], k0 + 1);
+    }
   }
-  issue_bred(br[1], k0 - 1);
-  issue_bred(br[0], k0);
-  issue_bblk(bk[0], k0);
+  if (PF == 2) {
+    issue_bred(br[1 % PF], k0 - 1);
+    issue_bred(br[0], k0);
+    issue_bblk(bk[0], k0);
+  } else {
+    issue_bred(br[0], k0 - 1);
+  }
 
-  // one plane step; P = (m - k0) & 1 selects the register buffers
+  // One plane step.  Loads for plane q are issued PF steps before plane q is
+  // used; with PF = 2 the buffers alternate by plane parity
+  // (P = (m - k0) & 1, compile-time through the two-step unrolled loop).
   auto step = [&](int m, auto Pc) {
-    constexpr int P = decltype(Pc)::value;
+    constexpr int P = PF == 2 ? decltype(Pc)::value : 0;
+    constexpr int PB = PF == 2 ? 1 - P : 0;
     lds_barrier();
     red_plane(br[P], m);                     // needs b(m), x planes m-1..m+1
-    issue_bred(br[P], m + 2);
+    issue_bred(br[P], m + PF);
     lds_barrier();
-    if (m - 1 >= k0) black_plane(bk[1 - P], m - 1);
-    if (m + 1 >= k0 + 1) issue_bblk(bk[1 - P], m + 1);
+    if (m - 1 >= k0) black_plane(bk[PB], m - 1);
+    if (m >= k0) issue_bblk(bk[PB], m - 1 + PF);
     if (MODE != 1) {
       stage_x(xp[P], m + 2);
-      issue_x(xp[P], m + 4);
+      issue_x(xp[P], m + 2 + PF);
     }
   };
   for (int m = k0 - 1; m <= k1; m += 2) {
@@ -317,15 +328,16 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
   }
 }
 
-template <typename T, int TX, int TY, int MODE>
-__global__ void __launch_bounds__(FUSED_THREADS)
-k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G) {
+// One block of the fused iteration; bid numbers the blocks of one grid.
+template <typename T, int TX, int TY, int MODE, int PF>
+__device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, const T* __restrict__ xi,
+                                               const T* __restrict__ b, const Lvl<T>& L, T omega,
+                                               const FusedGrid& G) {
   using S = FusedShape<T, TX, TY>;
   __shared__ T sa[4 * S::PLANE];   // x_in, first cell of each pair (i = i0-2+2p, odd)
   __shared__ T sb[4 * S::PLANE];   // x_in, second cell (i+1)
   __shared__ T sr[3 * S::QY * S::RP];   // new red value of each pair
 
-  const int bid = blockIdx.x;
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
   const int i0 = 1 + (rxy % G.gx) * TX;
@@ -334,9 +346,15 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
   const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
   const bool in = i0 >= 3 && i0 + TX + 1 <= L.nx && j0 >= 3 && j0 + TY + 1 <= L.ny;
   if (in)
-    rb_fused_body<T, TX, TY, MODE, true>(xo, xi, b, L, omega, i0, j0, k0, k1, sa, sb, sr);
+    rb_fused_body<T, TX, TY, MODE, PF, true>(xo, xi, b, L, omega, i0, j0, k0, k1, sa, sb, sr);
   else
-    rb_fused_body<T, TX, TY, MODE, false>(xo, xi, b, L, omega, i0, j0, k0, k1, sa, sb, sr);
+    rb_fused_body<T, TX, TY, MODE, PF, false>(xo, xi, b, L, omega, i0, j0, k0, k1, sa, sb, sr);
+}
+
+template <typename T, int TX, int TY, int MODE, int PF>
+__global__ void __launch_bounds__(FUSED_THREADS)
+k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G) {
+  rb_fused_block<T, TX, TY, MODE, PF>(blockIdx.x, xo, xi, b, L, omega, G);
 }
 
 }  // namespace pepsmg

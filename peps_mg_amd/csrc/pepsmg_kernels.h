@@ -53,6 +53,16 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Load of a read-only array element at a wave-uniform index (per-plane z
+// coefficients and weights) through the constant address space, so it is
+// a scalar load: it waits on the scalar/LDS counter, not on the vector
+// memory counter that also tracks the plane prefetches in flight.  Only for
+// arrays no kernel writes.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, int i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 template <typename T>
 __device__ __forceinline__ long lidx(const Lvl<T>& L, int i, int j, int k) {
   return (long)(k - 1) * L.pxy + (long)(j - 1) * L.px + (i - 1);

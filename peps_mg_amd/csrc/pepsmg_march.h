@@ -92,9 +92,10 @@ __device__ __forceinline__ T march_resid(T (*s0)[RX], T (*sm)[RX], T (*sp)[RX], 
 }
 
 template <typename T, int TY, int KIND>
-__global__ void __launch_bounds__(MARCH_THREADS)
-k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __restrict__ b, T* __restrict__ rout,
-              float* __restrict__ r32, T* __restrict__ xnew, Lvl<T> L, MarchGrid G, double* __restrict__ partial) {
+__device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__ x, const float* __restrict__ e,
+                                                  const T* __restrict__ b, T* __restrict__ rout,
+                                                  float* __restrict__ r32, T* __restrict__ xnew, const Lvl<T>& L,
+                                                  const MarchGrid& G, double* __restrict__ partial) {
   constexpr bool MIXED = (KIND == 2);
   using St = XStager<T, TY, MIXED>;
   constexpr int RX = St::RX, RY = St::RY;
@@ -103,7 +104,6 @@ k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __r
   __shared__ T sx[4][RY][RX];
   __shared__ double sh[MARCH_THREADS / 64];
 
-  const int bid = blockIdx.x;
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
   const int i0 = 1 + (rxy % G.gx) * 64;
@@ -161,7 +161,7 @@ k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __r
     T(*s0)[RX] = sx[m & 3];
     T(*sm)[RX] = sx[(m - 1) & 3];
     T(*sp)[RX] = sx[(m + 1) & 3];
-    T c5 = L.cwz[m], c6 = L.cez[m];
+    T c5 = ldc(L.cwz, m), c6 = ldc(L.cez, m);
     V2 bv[NP];
 #pragma unroll
     for (int q = 0; q < NP; q++) bv[q] = bpre[q];
@@ -209,13 +209,20 @@ k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __r
   if (threadIdx.x == 0) partial[bid] = s;
 }
 
+template <typename T, int TY, int KIND>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __restrict__ b, T* __restrict__ rout,
+              float* __restrict__ r32, T* __restrict__ xnew, Lvl<T> L, MarchGrid G, double* __restrict__ partial) {
+  resid_march_block<T, TY, KIND>(blockIdx.x, x, e, b, rout, r32, xnew, L, G, partial);
+}
+
 // Residual + restriction for a level that halves every axis.  Fine tile
 // 64 x 16, coarse tile 32 x 8 (one coarse cell per thread); chunks hold an
 // even number of fine planes.
 template <typename T>
-__global__ void __launch_bounds__(MARCH_THREADS)
-k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restrict__ b, Lvl<T> L, Lvl<T> Cc,
-                 RestrictW<T> W, MarchGrid G) {
+__device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc, const T* __restrict__ x,
+                                                     const T* __restrict__ b, const Lvl<T>& L, const Lvl<T>& Cc,
+                                                     const RestrictW<T>& W, const MarchGrid& G) {
   constexpr int TY = 16;
   using St = XStager<T, TY, false>;
   constexpr int RX = St::RX, RY = St::RY;
@@ -224,7 +231,6 @@ k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restric
   __shared__ T sx[4][RY][RX];
   __shared__ T sres[2][TY][64];
 
-  const int bid = blockIdx.x;
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
   const int i0 = 1 + (rxy % G.gx) * 64;
@@ -289,7 +295,7 @@ k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restric
     T(*s0)[RX] = sx[m & 3];
     T(*sm)[RX] = sx[(m - 1) & 3];
     T(*sp)[RX] = sx[(m + 1) & 3];
-    T c5 = L.cwz[m], c6 = L.cez[m];
+    T c5 = ldc(L.cwz, m), c6 = ldc(L.cez, m);
     V2 bv[NP];
 #pragma unroll
     for (int q = 0; q < NP; q++) bv[q] = bpre[q];
@@ -312,7 +318,7 @@ k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restric
       lds_barrier();
       int kc = m / 2;
       if (cact) {
-        T az1 = W.wz[m - 1], az2 = W.wz[m];
+        T az1 = ldc(W.wz, m - 1), az2 = ldc(W.wz, m);
         T v111 = ax1 * ay1 * az1, v211 = ax2 * ay1 * az1, v121 = ax1 * ay2 * az1, v221 = ax2 * ay2 * az1;
         T v112 = ax1 * ay1 * az2, v212 = ax2 * ay1 * az2, v122 = ax1 * ay2 * az2, v222 = ax2 * ay2 * az2;
         T vol_c = v111 + v211 + v121 + v221 + v112 + v212 + v122 + v222;
@@ -325,6 +331,13 @@ k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restric
       }
     }
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restrict__ b, Lvl<T> L, Lvl<T> Cc,
+                 RestrictW<T> W, MarchGrid G) {
+  restrict_march_block<T>(blockIdx.x, bc, x, b, L, Cc, W, G);
 }
 
 }  // namespace pepsmg

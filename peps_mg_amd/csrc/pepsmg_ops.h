@@ -1,0 +1,198 @@
+// pepsmg_ops.h -- host-side launch helpers shared by the single-grid and
+// batched solvers: block decompositions of the z-march kernels and one
+// launcher per operation on one level.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "pepsmg_coarse.h"
+#include "pepsmg_fused.h"
+#include "pepsmg_internal.h"
+#include "pepsmg_march.h"
+
+namespace pepsmg {
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+static inline dim3 grid3(int nx, int ny, int nz, int bx, int by) {
+  return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)((ny + by - 1) / by), (unsigned)nz);
+}
+
+template <typename T>
+struct Ops {
+  hipStream_t s;
+  int tile_y = 8;
+
+  // Block decomposition of the fused smoother: 64 x TY tiles, z-chunks of
+  // at least 4 planes, about 2048 blocks, chunk count a multiple of 8 when
+  // there are 8 or more (one z-slab per XCD).
+  // x tile width of the fused smoother: 64 cells in FP64, 128 in FP32 (same
+  // bytes per row)
+  static constexpr int FTX = sizeof(T) == 8 ? 64 : 128;
+
+  static FusedGrid fused_grid(const DevLevel<T>& d, int TX, int TY) {
+    FusedGrid G;
+    G.gx = (d.nx + TX - 1) / TX;
+    G.gy = (d.ny + TY - 1) / TY;
+    long nxy = (long)G.gx * G.gy;
+    long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + 3) / 4));
+    if (nch >= 8) nch = (nch + 7) / 8 * 8;
+    G.kchunk = (int)((d.nz + nch - 1) / nch);
+    G.nch = (d.nz + G.kchunk - 1) / G.kchunk;
+    return G;
+  }
+
+  template <int TY, int MODE, int PF>
+  void fused_launch(DevLevel<T>& d, T omega) {
+    FusedGrid G = fused_grid(d, FTX, TY);
+    unsigned nb = (unsigned)(G.gx * G.gy * G.nch);
+    hipLaunchKernelGGL((k_rb_fused<T, FTX, TY, MODE, PF>), dim3(nb), dim3(FUSED_THREADS), 0, s, d.x2, d.x, d.b,
+                       d.L(), omega, G);
+  }
+
+  template <int TY, int PF>
+  void fused_mode(DevLevel<T>& d, T omega, int mode) {
+    if (mode == 1) fused_launch<TY, 1, PF>(d, omega);
+    else fused_launch<TY, 0, PF>(d, omega);
+  }
+
+  // One fused red+black iteration x -> x2, then swap.  mode 0 plain, 1 zero
+  // start (x taken as 0).  prefetch: planes of look-ahead of the global loads.
+  int prefetch = 1;
+  void fused_iter(DevLevel<T>& d, T omega, int mode) {
+    if (tile_y == 16) {
+      if (prefetch == 2) fused_mode<16, 2>(d, omega, mode);
+      else fused_mode<16, 1>(d, omega, mode);
+    } else {
+      if (prefetch == 2) fused_mode<8, 2>(d, omega, mode);
+      else fused_mode<8, 1>(d, omega, mode);
+    }
+    std::swap(d.x, d.x2);
+  }
+
+  void smooth_colour(DevLevel<T>& d, const T* b, T* x, T omega, int colour, bool zero) {
+    int hx = (d.nx + 1) / 2;
+    dim3 g = grid3(hx, d.ny, d.nz, 64, 4), blk(64, 4, 1);
+    if (zero)
+      hipLaunchKernelGGL((k_rb_colour<T, true>), g, blk, 0, s, x, b, d.L(), omega, colour);
+    else
+      hipLaunchKernelGGL((k_rb_colour<T, false>), g, blk, 0, s, x, b, d.L(), omega, colour);
+  }
+
+  // nu red/black iterations on (x, b); zero_start: x is taken as 0 on entry
+  void rbgs(DevLevel<T>& d, T* x, const T* b, int nu, T omega, bool zero_start) {
+    if (nu <= 0) {
+      if (zero_start) hipMemsetAsync(x, 0, d.ncell * sizeof(T), s);
+      return;
+    }
+    for (int it = 0; it < nu; it++) {
+      smooth_colour(d, b, x, omega, 0, zero_start && it == 0);
+      smooth_colour(d, b, x, omega, 1, false);
+    }
+  }
+
+  void residual(DevLevel<T>& d, T* r, const T* x, const T* b) {
+    hipLaunchKernelGGL((k_residual<T>), grid3(d.nx, d.ny, d.nz, 64, 4), dim3(64, 4, 1), 0, s, r, x, b, d.L());
+  }
+
+  void apply(DevLevel<T>& d, T* y, const T* x) {
+    hipLaunchKernelGGL((k_apply<T>), grid3(d.nx, d.ny, d.nz, 64, 4), dim3(64, 4, 1), 0, s, y, x, d.L());
+  }
+
+  // z-march decomposition: 64 x TY tiles, about 2048 blocks, chunk count a
+  // multiple of 8 when >= 8; even chunks for the restriction kernel.
+  static MarchGrid march_grid(const DevLevel<T>& d, int TY, bool even) {
+    MarchGrid G;
+    G.gx = (d.nx + 63) / 64;
+    G.gy = (d.ny + TY - 1) / TY;
+    long nxy = (long)G.gx * G.gy;
+    long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + 3) / 4));
+    if (nch >= 8) nch = (nch + 7) / 8 * 8;
+    int kc = (int)((d.nz + nch - 1) / nch);
+    if (even && (kc & 1)) kc++;
+    G.kchunk = kc;
+    G.nch = (d.nz + kc - 1) / kc;
+    return G;
+  }
+
+  void residual_restrict(DevLevel<T>& f, DevLevel<T>& c, const T* x, const T* b) {
+    if (f.rs[0] == 2 && f.rs[1] == 2 && f.rs[2] == 2) {
+      MarchGrid G = march_grid(f, 16, true);
+      hipLaunchKernelGGL((k_restrict_march<T>), dim3((unsigned)(G.gx * G.gy * G.nch)), dim3(MARCH_THREADS), 0, s,
+                         c.b, x, b, f.L(), c.L(), f.RW(), G);
+      return;
+    }
+    hipLaunchKernelGGL((k_restrict<T, true>), grid3(c.nx, c.ny, c.nz, 64, 4), dim3(64, 4, 1), 0, s, c.b, x, b,
+                       f.L(), c.L(), f.RW());
+  }
+
+  // |b - A x|^2 into out[0] (partial sums per block, then one block); r
+  // stored when non-null.
+  void resid_norm_march(DevLevel<T>& d, const T* x, const T* b, T* r, double* partial, double* out) {
+    MarchGrid G = march_grid(d, 8, false);
+    int nb = G.gx * G.gy * G.nch;
+    if (nb > PARTIAL_CAP) {
+      resid_norm(d, x, b, r, partial, out);
+      return;
+    }
+    if (r)
+      hipLaunchKernelGGL((k_resid_march<T, 8, 1>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, s, x,
+                         (const float*)nullptr, b, r, (float*)nullptr, (T*)nullptr, d.L(), G, partial);
+    else
+      hipLaunchKernelGGL((k_resid_march<T, 8, 0>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, s, x,
+                         (const float*)nullptr, b, (T*)nullptr, (float*)nullptr, (T*)nullptr, d.L(), G, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+  }
+
+  // prolongation march: 64 x 16 fine tiles, chunks of coarse planes
+  static MarchGrid prolong_grid(const DevLevel<T>& f, const DevLevel<T>& c) {
+    MarchGrid G;
+    G.gx = (f.nx + 63) / 64;
+    G.gy = (f.ny + 15) / 16;
+    long nxy = (long)G.gx * G.gy;
+    long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (c.nz + 1) / 2));
+    if (nch >= 8) nch = (nch + 7) / 8 * 8;
+    G.kchunk = (int)((c.nz + nch - 1) / nch);
+    G.nch = (c.nz + G.kchunk - 1) / G.kchunk;
+    return G;
+  }
+
+  void prolong_add(DevLevel<T>& f, DevLevel<T>& c, T* xf) {
+    if (f.ps[0] == 2 && f.ps[1] == 2 && f.ps[2] == 2) {
+      MarchGrid G = prolong_grid(f, c);
+      hipLaunchKernelGGL((k_prolong_march<T>), dim3((unsigned)(G.gx * G.gy * G.nch)), dim3(MARCH_THREADS), 0, s, xf,
+                         c.x, f.L(), c.L(), f.PW(), G);
+      return;
+    }
+    hipLaunchKernelGGL((k_prolong_add<T>), grid3(f.nx, f.ny, f.nz, 64, 4), dim3(64, 4, 1), 0, s, xf, c.x, f.L(),
+                       c.L(), f.PW());
+  }
+
+  int red_blocks(const DevLevel<T>& d) {
+    long n = d.cells();
+    long b = (n + RED_THREADS - 1) / RED_THREADS;
+    return (int)std::max(1L, std::min<long>(b, RED_BLOCKS));
+  }
+
+  // out[0] = sum x*y over the interior
+  void dot(DevLevel<T>& d, const T* x, const T* y, double* partial, double* out) {
+    int nb = red_blocks(d);
+    hipLaunchKernelGGL((k_dot_partial<T, 0, false>), dim3(nb), dim3(RED_THREADS), 0, s, x, y, d.L(), partial,
+                       (T*)nullptr);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+  }
+
+  // out[0] = |b - A x|^2; optionally r = b - A x
+  void resid_norm(DevLevel<T>& d, const T* x, const T* b, T* r, double* partial, double* out) {
+    int nb = red_blocks(d);
+    if (r)
+      hipLaunchKernelGGL((k_dot_partial<T, 1, true>), dim3(nb), dim3(RED_THREADS), 0, s, x, b, d.L(), partial, r);
+    else
+      hipLaunchKernelGGL((k_dot_partial<T, 1, false>), dim3(nb), dim3(RED_THREADS), 0, s, x, b, d.L(), partial,
+                         (T*)nullptr);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+  }
+};
+
+}  // namespace pepsmg

@@ -17,9 +17,9 @@ ny = {n}
 nz = {n}
 /
 &origin
-ox = 0.0d0
-oy = 0.0d0
-oz = 0.0d0
+ox = {o}
+oy = {o}
+oz = {o}
 /
 &length
 lx = 1.0d0
@@ -60,8 +60,8 @@ def ensure_built():
     return EXE
 
 
-def write_input(d, n=16, lz="1.0d0", npx=1, nl=2, aggr=0, gpu=None):
-    txt = PARA.format(n=n, lz=lz, npx=npx, nl=nl, aggr=aggr)
+def write_input(d, n=16, lz="1.0d0", npx=1, nl=2, aggr=0, gpu=None, o="0.0d0"):
+    txt = PARA.format(n=n, lz=lz, npx=npx, nl=nl, aggr=aggr, o=o)
     if gpu:
         txt += "&gpu\n" + "\n".join(f"{k} = {v}" for k, v in gpu.items()) + "\n/\n"
     with open(os.path.join(d, "PARA_INPUT.inp"), "w") as f:
@@ -115,9 +115,10 @@ def test_driver_matches_oracle(tmp_path):
 
 @pytest.mark.gpu
 def test_driver_cosine_problem(tmp_path):
-    """The reference's Problem 1 (cosine RHS) converges with O(h^2) RMS error."""
+    """The reference's Problem 1 (cosine RHS over -l/2 <= x,y,z <= l/2)
+    converges with O(h^2) RMS error."""
     exe = ensure_built()
-    write_input(tmp_path, n=32, nl=3)
+    write_input(tmp_path, n=32, nl=3, o="-0.5d0")
     p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stdout + p.stderr
     rms = float(re.search(r"RMS =\s+(\S+)", p.stdout).group(1))

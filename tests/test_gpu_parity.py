@@ -121,11 +121,13 @@ def run_both(dims, lens, stretch, nlevel, nu, omega, ncycle=60, smoother=0, seed
     return (xo, ro, nc, hist, cit, res0), (xg, rg, res)
 
 
-SMOOTHER_PATHS = [dict(fused_smoother=1, tile_y=8), dict(fused_smoother=1, tile_y=16), dict(fused_smoother=0),
+SMOOTHER_PATHS = [dict(fused_smoother=1, tile_y=8), dict(fused_smoother=1, tile_y=16), dict(fused_smoother=2, tile_y=8),
+                  dict(fused_smoother=2, tile_y=16), dict(fused_smoother=0),
                   dict(fused_smoother=1, coarse_switch=0), dict(fused_smoother=1, coarse_switch=100000)]
 
 
-@pytest.mark.parametrize("opts", SMOOTHER_PATHS, ids=["fused8", "fused16", "colour", "nopersist", "persist-big"])
+@pytest.mark.parametrize("opts", SMOOTHER_PATHS, ids=["fused8", "fused16", "fused8-pf2", "fused16-pf2", "colour",
+                                                      "nopersist", "persist-big"])
 @pytest.mark.parametrize("dims,lens,stretch,nlevel,nu,omega", VCASES)
 def test_vcycle_fp64_bitwise(dims, lens, stretch, nlevel, nu, omega, opts):
     (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both(dims, lens, stretch, nlevel, nu, omega, **opts)
