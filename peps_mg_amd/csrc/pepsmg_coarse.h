@@ -62,40 +62,43 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
     njr[q] = (jodd ? jc - 1 : jc + 1) - jc0 + 1;
   }
 
+  // Coarse and fine planes load from clamped in-bounds addresses with no
+  // branch around the loads; coarse values outside the domain are zeroed
+  // when staged, fine values of cells outside are never stored.
   T cpre[NCL];
+  int cq = 0;
   auto issue_c = [&](int kc) {
+    cq = kc;
+    const int kcc = min(max(kc, 1), Cc.nz);
 #pragma unroll
     for (int q = 0; q < NCL; q++) {
-      int e = t + q * MARCH_THREADS;
-      T v = T(0);
-      if (e < CX * CY && kc >= 1 && kc <= Cc.nz) {
-        int cy = e / CX, cx = e - cy * CX;
-        int gi = ic0 - 1 + cx, gj = jc0 - 1 + cy;
-        if (gi >= 1 && gi <= Cc.nx && gj >= 1 && gj <= Cc.ny) v = xc[lidx(Cc, gi, gj, kc)];
-      }
-      cpre[q] = v;
+      int e = min(t + q * MARCH_THREADS, CX * CY - 1);
+      int cy = e / CX, cx = e - cy * CX;
+      int gi = min(max(ic0 - 1 + cx, 1), Cc.nx), gj = min(max(jc0 - 1 + cy, 1), Cc.ny);
+      cpre[q] = xc[lidx(Cc, gi, gj, kcc)];
     }
   };
   auto stage_c = [&](int kc) {
+    const bool vz = cq >= 1 && cq <= Cc.nz;
 #pragma unroll
     for (int q = 0; q < NCL; q++) {
       int e = t + q * MARCH_THREADS;
-      if (e < CX * CY) (&sc[kc & 3][0][0])[e] = cpre[q];
+      if (e < CX * CY) {
+        int cy = e / CX, cx = e - cy * CX;
+        int gi = ic0 - 1 + cx, gj = jc0 - 1 + cy;
+        bool v = vz && gi >= 1 && gi <= Cc.nx && gj >= 1 && gj <= Cc.ny;
+        (&sc[kc & 3][0][0])[e] = v ? cpre[q] : T(0);
+      }
     }
   };
   V2 xpre[NP];
   auto issue_f = [&](int f) {
+    const int fc = min(max(f, 1), F.nz);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
-      V2 v;
-      v.x = T(0);
-      v.y = T(0);
-      if (act[q] && f >= 1 && f <= F.nz) {
-        long q0 = lidx(F, ia, j0 + ly[q], f);
-        if (two[q]) v = *reinterpret_cast<const V2*>(xf + q0);
-        else v.x = xf[q0];
-      }
-      xpre[q] = v;
+      // an odd column <= nx keeps the 2-element load inside the padded row
+      int gi = act[q] ? ia : 1, gj = act[q] ? j0 + ly[q] : 1;
+      xpre[q] = *reinterpret_cast<const V2*>(xf + lidx(F, gi, gj, fc));
     }
   };
 

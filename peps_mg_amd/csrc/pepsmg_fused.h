@@ -16,17 +16,15 @@
 // each a dense array over pairs) and the new red values as one value per
 // pair, so consecutive lanes touch consecutive LDS words; each lane picks
 // its slot by address, not by branch.  LDS holds four x_in planes with a
-// two-cell ring and three red planes with a one-cell ring.  Per plane m (two
+// two-cell ring and four red planes with a one-cell ring.  Per plane m (two
 // barriers):
 //   red(m) on tile+ring | barrier | black(m-1) on the tile and store of
 //   plane m-1; stage plane m+2 from registers into LDS | barrier
-// Global loads of x (plane m+4) and b (red plane m+2, black plane m+1) are
-// issued two plane steps before use into double-buffered registers (the
-// plane loop is unrolled by two so buffer indices are compile-time).
-// Blocks whose tile plus ring lies inside the domain (most of them) run a
-// specialisation without bounds guards.  Coefficients of a thread's fixed
-// pairs are kept in registers.
-//
+// Global loads of x (plane m+2+PF) and b (red plane m+PF, black plane
+// m-1+PF) are issued PF plane steps before use (PF = 1 or 2, registers
+// double-buffered for 2).  Per-thread data that depends on the plane only
+// through its parity (which cell of a pair is red, coefficients, partial
+// diagonal sums, LDS and b offsets) is precomputed for both parities.
 // Blocks are numbered with the z-chunk fastest, so with 8 (or a multiple of
 // 8) chunks each XCD (blocks are dealt round-robin over the 8 XCDs) owns
 // whole z-slabs and the tile rings are re-read from its own L2.
@@ -55,13 +53,13 @@ struct FusedGrid {
 
 // omega * (b - temp) / c0 + (1 - omega) * xold, the reference's SOR update
 // (src/rbgs_poisson_matrix.f90:250).  FP64 divides exactly (bit-identical
-// results); the FP32 inner solver of the mixed-precision path uses the
-// hardware reciprocal-based divide.
+// results); the FP32 inner solver of the mixed-precision path multiplies by
+// the hardware reciprocal (1 ulp), which leaves its convergence unchanged.
 __device__ __forceinline__ double sor_update(double omega, double bv, double temp, double c0, double xold) {
   return omega * (bv - temp) / c0 + (1.0 - omega) * xold;
 }
 __device__ __forceinline__ float sor_update(float omega, float bv, float temp, float c0, float xold) {
-  return __fdividef(omega * (bv - temp), c0) + (1.0f - omega) * xold;
+  return omega * (bv - temp) * __builtin_amdgcn_rcpf(c0) + (1.0f - omega) * xold;
 }
 
 template <typename T, int TX, int TY>
@@ -77,212 +75,207 @@ struct FusedShape {
   static constexpr int PLANE = RY * RP;
 };
 
-// IN: the tile and its two-cell ring lie inside the domain in x and y.
-template <typename T, int TX, int TY, int MODE, int PF, bool IN>
+// Everything that depends on a thread's fixed pairs is computed once per
+// block, for both plane parities z: which cell of a pair is red (black) on
+// planes of parity z, its x coefficients, the partial diagonal sum
+// ((c1 + c2) + c3) + c4 (c0 = -((s4 + c5) + c6) is the reference's
+// left-to-right sum), its LDS offsets and the offset of its b value.  The
+// chunk start k0 is odd (even chunk length), so each plane step of the
+// two-step unrolled loop knows its plane parity at compile time.
+template <typename T, int TX, int TY, int MODE, int PF>
 __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b,
                                               const Lvl<T>& L, T omega, int i0, int j0, int k0, int k1,
-                                              T* __restrict__ sa, T* __restrict__ sb, T* __restrict__ sr) {
+                                              T* __restrict__ sx, T* __restrict__ sr) {
   using S = FusedShape<T, TX, TY>;
   constexpr int NT = S::NT, RP = S::RP, RY = S::RY, QY = S::QY, TP = S::TP;
   constexpr int NXL = S::NXL, NRED = S::NRED, NBLK = S::NBLK, PLANE = S::PLANE;
+  constexpr int SB = 4 * PLANE;           // second-cell slots follow the four first-cell slots
+  constexpr int QP = QY * RP;             // one red plane
   using V2 = typename Vec2<T>::type;
   const int t = threadIdx.x;
+  const long pxy = L.pxy;
 
-  // ---- red pairs of this thread: ring row qy, pair p ----
-  int rq_y[NRED], rq_p[NRED];
-  bool rq_ina[NRED], rq_inb[NRED];
-  T rc_wa[NRED], rc_ea[NRED], rc_wb[NRED], rc_eb[NRED], rc_wy[NRED], rc_ey[NRED];
+  // ---- red pairs: ring row qy (y = j0-1+qy), pair p (x = i0-2+2p, +1) ----
+  bool r_in[NRED][2];
+  T r_c1[NRED][2], r_c2[NRED][2], r_s4[NRED][2], r_c3[NRED], r_c4[NRED];
+  int r_ss[NRED][2], r_oo[NRED][2], r_bo[NRED][2], r_q[NRED];
 #pragma unroll
   for (int q = 0; q < NRED; q++) {
-    int e = t + q * NT;
+    int e = min(t + q * NT, QP - 1);
     int qy = e / RP, p = e - qy * RP;
-    rq_y[q] = qy;
-    rq_p[q] = p;
     int gj = j0 - 1 + qy, ia = i0 - 2 + 2 * p;
-    bool row = e < QY * RP && (IN || (gj >= 1 && gj <= L.ny));
+    bool row = t + q * NT < QP && gj >= 1 && gj <= L.ny;
     // the first cell is used only from column i0-1 on, the second up to i0+TX
-    rq_ina[q] = row && p > 0 && (IN || (ia >= 1 && ia <= L.nx));
-    rq_inb[q] = row && p < RP - 1 && (IN || (ia + 1 >= 1 && ia + 1 <= L.nx));
-    rc_wa[q] = rq_ina[q] ? L.cwx[ia] : T(0);
-    rc_ea[q] = rq_ina[q] ? L.cex[ia] : T(0);
-    rc_wb[q] = rq_inb[q] ? L.cwx[ia + 1] : T(0);
-    rc_eb[q] = rq_inb[q] ? L.cex[ia + 1] : T(0);
-    rc_wy[q] = row ? L.cwy[gj] : T(0);
-    rc_ey[q] = row ? L.cey[gj] : T(0);
+    bool ina = row && p > 0 && ia >= 1 && ia <= L.nx;
+    bool inb = row && p < RP - 1 && ia + 1 >= 1 && ia + 1 <= L.nx;
+    T c3 = row ? L.cwy[gj] : T(0), c4 = row ? L.cey[gj] : T(0);
+    r_c3[q] = c3;
+    r_c4[q] = c4;
+    r_q[q] = e;
+    int rrow = (qy + 1) * RP + p;           // x_in row of this ring row
+#pragma unroll
+    for (int z = 0; z < 2; z++) {
+      bool first = ((gj + z) & 1) == 0;     // red is the first cell iff gj + k even
+      bool in = first ? ina : inb;
+      int ic = first ? ia : ia + 1;
+      T c1 = in ? L.cwx[ic] : T(0), c2 = in ? L.cex[ic] : T(0);
+      r_in[q][z] = in;
+      r_c1[q][z] = c1;
+      r_c2[q][z] = c2;
+      r_s4[q][z] = c1 + c2 + c3 + c4;
+      r_ss[q][z] = (first ? 0 : SB) + rrow;
+      r_oo[q][z] = (first ? SB : 0) + rrow - (first ? 1 : 0);   // west neighbour; east is +1
+      r_bo[q][z] = in ? (gj - 1) * (int)L.px + (ic - 1) : 0;
+    }
   }
-  // ---- black pairs of this thread: tile row ty, pair p (1..TP) ----
-  int bq_y[NBLK], bq_p[NBLK];
-  bool bq_on[NBLK], bq_inb[NBLK];
-  T bc_wa[NBLK], bc_ea[NBLK], bc_wb[NBLK], bc_eb[NBLK], bc_wy[NBLK], bc_ey[NBLK];
+  // ---- black pairs: tile row ty (y = j0+ty), pair p = 1..TP ----
+  bool b_on[NBLK], b_odd[NBLK], b_in[NBLK][2];
+  T b_c1[NBLK][2], b_c2[NBLK][2], b_s4[NBLK][2], b_c3[NBLK], b_c4[NBLK];
+  int b_row[NBLK], b_ow[NBLK][2], b_xc[NBLK][2], b_bo[NBLK][2], b_so[NBLK];
 #pragma unroll
   for (int q = 0; q < NBLK; q++) {
-    int e = t + q * NT;
+    int e = min(t + q * NT, TY * TP - 1);
     int ty = e / TP, p = 1 + (e - ty * TP);
-    bq_y[q] = ty;
-    bq_p[q] = p;
     int gj = j0 + ty, ia = i0 - 2 + 2 * p;
-    bq_on[q] = e < TY * TP && (IN || (gj <= L.ny && ia <= L.nx));
-    bq_inb[q] = bq_on[q] && (IN || ia + 1 <= L.nx);
-    bc_wa[q] = bq_on[q] ? L.cwx[ia] : T(0);
-    bc_ea[q] = bq_on[q] ? L.cex[ia] : T(0);
-    bc_wb[q] = bq_inb[q] ? L.cwx[ia + 1] : T(0);
-    bc_eb[q] = bq_inb[q] ? L.cex[ia + 1] : T(0);
-    bc_wy[q] = bq_on[q] ? L.cwy[gj] : T(0);
-    bc_ey[q] = bq_on[q] ? L.cey[gj] : T(0);
+    bool on = t + q * NT < TY * TP && gj <= L.ny && ia <= L.nx;
+    bool two = on && ia + 1 <= L.nx;
+    b_on[q] = on;
+    b_odd[q] = (gj & 1) == 1;
+    T c3 = on ? L.cwy[gj] : T(0), c4 = on ? L.cey[gj] : T(0);
+    b_c3[q] = c3;
+    b_c4[q] = c4;
+    b_row[q] = (ty + 1) * RP + p;           // red-ring row of tile row ty
+    b_so[q] = on ? (gj - 1) * (int)L.px + (ia - 1) : 0;
+#pragma unroll
+    for (int z = 0; z < 2; z++) {
+      bool bfirst = ((gj + z) & 1) == 1;    // black is the first cell iff gj + k odd
+      bool in = bfirst ? on : two;
+      int ic = bfirst ? ia : ia + 1;
+      T c1 = in ? L.cwx[ic] : T(0), c2 = in ? L.cex[ic] : T(0);
+      b_in[q][z] = in;
+      b_c1[q][z] = c1;
+      b_c2[q][z] = c2;
+      b_s4[q][z] = c1 + c2 + c3 + c4;
+      b_ow[q][z] = b_row[q] - (bfirst ? 1 : 0);   // red x neighbours: pairs p-1,p or p,p+1
+      b_xc[q][z] = (bfirst ? 0 : SB) + (ty + 2) * RP + p;
+      b_bo[q][z] = in ? (gj - 1) * (int)L.px + (ic - 1) : 0;
+    }
+  }
+  // ---- x_in staging: pair e of a plane (row ry, pair p) ----
+  int x_off[NXL];
+  unsigned x_ok = 0;                        // bit 2q: first cell of item q inside, 2q+1: second
+#pragma unroll
+  for (int q = 0; q < NXL; q++) {
+    int e = min(t + q * NT, PLANE - 1);
+    int ry = e / RP, p = e - ry * RP;
+    int gi = i0 - 2 + 2 * p, gj = j0 - 2 + ry;    // gi is odd
+    bool vrow = gj >= 1 && gj <= L.ny;
+    if (vrow && gi >= 1 && gi <= L.nx) x_ok |= 1u << (2 * q);
+    if (vrow && gi + 1 >= 1 && gi + 1 <= L.nx) x_ok |= 2u << (2 * q);
+    // an odd gi <= nx keeps the 2-element load inside the padded row
+    int gic = (gi >= 1 && gi <= L.nx) ? gi : 1;
+    int gjc = min(max(gj, 1), L.ny);
+    x_off[q] = (gjc - 1) * (int)L.px + (gic - 1);
   }
 
-  // ---- x_in plane loads (pairs); PF buffers (indexed by plane parity when PF = 2) ----
+  // Loads are issued unconditionally (clamped plane, precomputed in-bounds
+  // offsets); values outside the domain are zeroed when staged or never
+  // read.  With no branch or select next to a load the compiler waits only
+  // for the loads a plane needs instead of draining the prefetches.
   V2 xp[PF][NXL];
-  // Loads are issued unconditionally from clamped in-bounds addresses; the
-  // values that lie outside the domain are zeroed only when they are used
-  // (stage_x) or never read (b of cells outside the domain).  With no branch
-  // or select right after a load the compiler waits on exactly the loads a
-  // plane needs (vmcnt(n) with n > 0) instead of draining the prefetches.
+  T br[PF][NRED], bk[PF][NBLK];
   auto issue_x = [&](V2* dst, int m) {
-    const int mc = min(max(m, 1), L.nz);
+    const T* base = xi + (long)(min(max(m, 1), L.nz) - 1) * pxy;
 #pragma unroll
-    for (int q = 0; q < NXL; q++) {
-      int e = t + q * NT;
-      int ec = min(e, RY * RP - 1);
-      int ry = ec / RP, p = ec - ry * RP;
-      int gi = i0 - 2 + 2 * p, gj = j0 - 2 + ry;    // gi is odd
-      // an odd gi <= nx keeps the 2-element load inside the padded row
-      int gic = IN ? gi : ((gi >= 1 && gi <= L.nx) ? gi : 1);
-      int gjc = IN ? gj : min(max(gj, 1), L.ny);
-      dst[q] = *reinterpret_cast<const V2*>(xi + lidx(L, gic, gjc, mc));
-    }
+    for (int q = 0; q < NXL; q++) dst[q] = *reinterpret_cast<const V2*>(base + x_off[q]);
   };
   auto stage_x = [&](const V2* src, int m) {
-    const bool vz = m >= 1 && m <= L.nz;
+    const unsigned ok = (m >= 1 && m <= L.nz) ? x_ok : 0u;
+    T* d = sx + (m & 3) * PLANE;
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
       int e = t + q * NT;
-      if (e < RY * RP) {
-        int ry = e / RP, p = e - ry * RP;
-        int gi = i0 - 2 + 2 * p, gj = j0 - 2 + ry;
-        bool vrow = vz && (IN || (gj >= 1 && gj <= L.ny));
-        bool va = vrow && (IN || (gi >= 1 && gi <= L.nx));
-        bool vb = vrow && (IN || (gi + 1 >= 1 && gi + 1 <= L.nx));
-        sa[(m & 3) * PLANE + e] = va ? src[q].x : T(0);
-        sb[(m & 3) * PLANE + e] = vb ? src[q].y : T(0);
+      if (e < PLANE) {
+        d[e] = ((ok >> (2 * q)) & 1u) ? src[q].x : T(0);
+        d[SB + e] = ((ok >> (2 * q)) & 2u) ? src[q].y : T(0);
       }
     }
   };
+  auto issue_bred = [&](T* dst, int m, auto Zc) {
+    constexpr int Z = decltype(Zc)::value;
+    const T* base = b + (long)(min(max(m, 1), L.nz) - 1) * pxy;
+#pragma unroll
+    for (int q = 0; q < NRED; q++) dst[q] = base[r_bo[q][Z]];
+  };
+  auto issue_bblk = [&](T* dst, int k, auto Zc) {
+    constexpr int Z = decltype(Zc)::value;
+    const T* base = b + (long)(min(max(k, 1), L.nz) - 1) * pxy;
+#pragma unroll
+    for (int q = 0; q < NBLK; q++) dst[q] = base[b_bo[q][Z]];
+  };
 
-  // red cell of pair (gj, m): the first cell (i odd) iff gj + m is even
-  T br[PF][NRED], bk[PF][NBLK];
-  // b of the red (black) cell of each pair; cells outside the domain load a
-  // clamped address and their value is never used
-  auto issue_bred = [&](T* dst, int m) {
+  // new red values of plane m (parity Z) on the tile and its ring
+  auto red_plane = [&](const T* bv, int m, auto Zc) {
+    constexpr int Z = decltype(Zc)::value;
+    const T* X0 = sx + (m & 3) * PLANE;
+    const T* Xm = sx + ((m - 1) & 3) * PLANE;
+    const T* Xp = sx + ((m + 1) & 3) * PLANE;
+    T* r = sr + (m & 3) * QP;
+    const bool zin = m >= 1 && m <= L.nz;
     const int mc = min(max(m, 1), L.nz);
+    const T c5 = ldc(L.cwz, mc), c6 = ldc(L.cez, mc);
 #pragma unroll
     for (int q = 0; q < NRED; q++) {
-      int gj = j0 - 1 + rq_y[q];
-      bool first = ((gj + m) & 1) == 0;
-      bool in = first ? rq_ina[q] : rq_inb[q];
-      int gi = i0 - 2 + 2 * rq_p[q] + (first ? 0 : 1);
-      int gic = in ? gi : 1, gjc = in ? gj : 1;
-      dst[q] = b[lidx(L, gic, gjc, mc)];
-    }
-  };
-  auto issue_bblk = [&](T* dst, int k) {
-    const int kc = min(max(k, 1), L.nz);
-#pragma unroll
-    for (int q = 0; q < NBLK; q++) {
-      int gj = j0 + bq_y[q];
-      bool bfirst = ((gj + k) & 1) == 1;     // black is the first cell iff gj + k odd
-      bool in = bfirst ? bq_on[q] : bq_inb[q];
-      int gi = i0 - 2 + 2 * bq_p[q] + (bfirst ? 0 : 1);
-      int gic = in ? gi : 1, gjc = in ? gj : 1;
-      dst[q] = b[lidx(L, gic, gjc, kc)];
-    }
-  };
-
-  auto red_plane = [&](const T* bv, int m) {
-    const int s0 = m & 3, sm = (m - 1) & 3, sp = (m + 1) & 3;
-    T* r = sr + ((m + 3) % 3) * (QY * RP);
-    bool zin = (m >= 1 && m <= L.nz);
-    T c5 = T(0), c6 = T(0);
-    if (zin) {
-      c5 = ldc(L.cwz, m);
-      c6 = ldc(L.cez, m);
-    }
-#pragma unroll
-    for (int q = 0; q < NRED; q++) {
-      int qy = rq_y[q], p = rq_p[q];
-      if (qy >= QY) continue;
-      int ry = qy + 1;                       // x_in row of this ring row
-      int gj = j0 - 1 + qy;
-      int first = ((gj + m) & 1) == 0;       // red cell is the first of its pair
+      if (q * NT >= QP) continue;
       T v = T(0);
-      if (zin && (first ? rq_ina[q] : rq_inb[q])) {
-        T c1 = first ? rc_wa[q] : rc_wb[q], c2 = first ? rc_ea[q] : rc_eb[q];
-        T c3 = rc_wy[q], c4 = rc_ey[q];
-        T c0 = diag(c1, c2, c3, c4, c5, c6);
-        // own slot SS (same column in other rows/planes), other slot OO
-        // (x neighbours: pairs p-1,p if first, else p,p+1)
-        const T* SS = first ? sa : sb;
-        const T* OO = first ? sb : sa;
-        int row = ry * RP + p;
-        int ow = s0 * PLANE + row - first;
-        T xw = OO[ow], xe = OO[ow + 1];
-        T xs = SS[s0 * PLANE + row - RP], xn = SS[s0 * PLANE + row + RP];
-        T xd = SS[sm * PLANE + row], xu = SS[sp * PLANE + row];
-        T xc = SS[s0 * PLANE + row];
-        T temp = c1 * xw + c2 * xe + c3 * xs + c4 * xn + c5 * xd + c6 * xu;
-        v = sor_update(omega, bv[q], temp, c0, xc);
+      if (zin && r_in[q][Z]) {
+        const int ss = r_ss[q][Z], oo = r_oo[q][Z];
+        T c1 = r_c1[q][Z], c2 = r_c2[q][Z], c3 = r_c3[q], c4 = r_c4[q];
+        T c0 = -(r_s4[q][Z] + c5 + c6);
+        T temp = c1 * X0[oo] + c2 * X0[oo + 1] + c3 * X0[ss - RP] + c4 * X0[ss + RP] + c5 * Xm[ss] + c6 * Xp[ss];
+        v = sor_update(omega, bv[q], temp, c0, X0[ss]);
       }
-      r[qy * RP + p] = v;
+      if (t + q * NT < QP) r[r_q[q]] = v;
     }
   };
 
-  auto black_plane = [&](const T* bv, int k) {
-    const T* r0 = sr + (k % 3) * (QY * RP);
-    const T* rm = sr + ((k + 2) % 3) * (QY * RP);
-    const T* rp = sr + ((k + 1) % 3) * (QY * RP);
-    const int s0 = k & 3;
-    T c5 = ldc(L.cwz, k), c6 = ldc(L.cez, k);
+  // black values of plane k (parity Z) on the tile, stored with the red ones
+  auto black_plane = [&](const T* bv, int k, auto Zc) {
+    constexpr int Z = decltype(Zc)::value;
+    const T* r0 = sr + (k & 3) * QP;
+    const T* rm = sr + ((k - 1) & 3) * QP;
+    const T* rp = sr + ((k + 1) & 3) * QP;
+    const T* X0 = sx + (k & 3) * PLANE;
+    const T c5 = ldc(L.cwz, k), c6 = ldc(L.cez, k);
+    T* out = xo + (long)(k - 1) * pxy;
 #pragma unroll
     for (int q = 0; q < NBLK; q++) {
-      if (!bq_on[q]) continue;
-      int ty = bq_y[q], p = bq_p[q];
-      int gj = j0 + ty;
-      int ry = ty + 1;                       // red row of tile row ty
-      int bfirst = ((gj + k) & 1) == 1;      // black cell is the first of its pair
-      bool in = bfirst ? true : bq_inb[q];
-      int row = ry * RP + p;
+      if (!b_on[q]) continue;
+      const int row = b_row[q];
       T vb = T(0);
-      if (in) {
-        T c1 = bfirst ? bc_wa[q] : bc_wb[q], c2 = bfirst ? bc_ea[q] : bc_eb[q];
-        T c3 = bc_wy[q], c4 = bc_ey[q];
-        T c0 = diag(c1, c2, c3, c4, c5, c6);
-        int ow = row - bfirst;               // red x neighbours: pairs p-1,p or p,p+1
-        T xw = r0[ow], xe = r0[ow + 1];
-        T temp = c1 * xw + c2 * xe + c3 * r0[row - RP] + c4 * r0[row + RP] + c5 * rm[row] + c6 * rp[row];
-        const T* SS = bfirst ? sa : sb;
-        T xc = SS[s0 * PLANE + (ty + 2) * RP + p];
-        vb = sor_update(omega, bv[q], temp, c0, xc);
+      if (b_in[q][Z]) {
+        const int ow = b_ow[q][Z];
+        T c1 = b_c1[q][Z], c2 = b_c2[q][Z], c3 = b_c3[q], c4 = b_c4[q];
+        T c0 = -(b_s4[q][Z] + c5 + c6);
+        T temp = c1 * r0[ow] + c2 * r0[ow + 1] + c3 * r0[row - RP] + c4 * r0[row + RP] + c5 * rm[row] + c6 * rp[row];
+        vb = sor_update(omega, bv[q], temp, c0, X0[b_xc[q][Z]]);
       }
       T vr = r0[row];
-      T lo = bfirst ? vb : vr, hi = bfirst ? vr : vb;
-      long q0 = lidx(L, i0 - 2 + 2 * p, gj, k);
-      if (bq_inb[q]) {
-        V2 w;
-        w.x = lo;
-        w.y = hi;
-        *reinterpret_cast<V2*>(xo + q0) = w;
-      } else {
-        xo[q0] = lo;
-      }
+      const bool bf = Z ? !b_odd[q] : b_odd[q];   // black is the first cell iff gj + k odd
+      // the pair is stored whole: a second cell beyond nx lies in the row's
+      // padding (odd column <= nx, pitch a multiple of 16) and gets 0
+      V2 w;
+      w.x = bf ? vb : vr;
+      w.y = bf ? vr : vb;
+      *reinterpret_cast<V2*>(out + b_so[q]) = w;
     }
   };
 
-  // prologue: planes k0-2 .. k0 staged; k0+1 .. k0+PF in flight
+  using Z0 = std::integral_constant<int, 0>;
+  using Z1 = std::integral_constant<int, 1>;
+  // prologue: planes k0-2 .. k0 staged; k0+1 .. k0+PF in flight (k0 is odd)
   if (MODE == 1) {
-    for (int e = t; e < 4 * PLANE; e += NT) {
-      sa[e] = T(0);
-      sb[e] = T(0);
-    }
+    for (int e = t; e < 8 * PLANE; e += NT) sx[e] = T(0);
   } else {
     issue_x(xp[0], k0 - 2);
     stage_x(xp[0], k0 - 2);
@@ -298,28 +291,29 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     }
   }
   if (PF == 2) {
-    issue_bred(br[1 % PF], k0 - 1);
-    issue_bred(br[0], k0);
-    issue_bblk(bk[0], k0);
+    issue_bred(br[1 % PF], k0 - 1, Z0());
+    issue_bred(br[0], k0, Z1());
+    issue_bblk(bk[0], k0, Z1());
   } else {
-    issue_bred(br[0], k0 - 1);
+    issue_bred(br[0], k0 - 1, Z0());
   }
 
-  // One plane step.  Loads for plane q are issued PF steps before plane q is
-  // used; with PF = 2 the buffers alternate by plane parity
-  // (P = (m - k0) & 1, compile-time through the two-step unrolled loop).
+  // One plane step for plane m: red(m) | barrier | black(m-1), stage x(m+2).
+  // P = (m - k0) & 1, so plane m has parity 1 - P.  Loads for a plane are
+  // issued PF steps before it is used; with PF = 2 the buffers alternate by P.
   auto step = [&](int m, auto Pc) {
-    constexpr int P = PF == 2 ? decltype(Pc)::value : 0;
-    constexpr int PB = PF == 2 ? 1 - P : 0;
+    constexpr int P = decltype(Pc)::value;
+    constexpr int ZM = 1 - P;                       // parity of m
+    constexpr int B = PF == 2 ? P : 0, BB = PF == 2 ? 1 - P : 0;
     lds_barrier();
-    red_plane(br[P], m);                     // needs b(m), x planes m-1..m+1
-    issue_bred(br[P], m + PF);
+    red_plane(br[B], m, std::integral_constant<int, ZM>());     // needs b(m), x planes m-1..m+1
+    issue_bred(br[B], m + PF, std::integral_constant<int, (ZM + PF) & 1>());
     lds_barrier();
-    if (m - 1 >= k0) black_plane(bk[PB], m - 1);
-    if (m >= k0) issue_bblk(bk[PB], m - 1 + PF);
+    if (m - 1 >= k0) black_plane(bk[BB], m - 1, std::integral_constant<int, 1 - ZM>());
+    if (m >= k0) issue_bblk(bk[BB], m - 1 + PF, std::integral_constant<int, (1 - ZM + PF) & 1>());
     if (MODE != 1) {
-      stage_x(xp[P], m + 2);
-      issue_x(xp[P], m + 2 + PF);
+      stage_x(xp[B], m + 2);
+      issue_x(xp[B], m + 2 + PF);
     }
   };
   for (int m = k0 - 1; m <= k1; m += 2) {
@@ -334,21 +328,18 @@ __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, cons
                                                const T* __restrict__ b, const Lvl<T>& L, T omega,
                                                const FusedGrid& G) {
   using S = FusedShape<T, TX, TY>;
-  __shared__ T sa[4 * S::PLANE];   // x_in, first cell of each pair (i = i0-2+2p, odd)
-  __shared__ T sb[4 * S::PLANE];   // x_in, second cell (i+1)
-  __shared__ T sr[3 * S::QY * S::RP];   // new red value of each pair
+  // x_in planes (4 slots): first cells of the pairs (i = i0-2+2p, odd),
+  // then second cells (i+1); new red values (4 slots), one per pair
+  __shared__ T sx[8 * S::PLANE];
+  __shared__ T sr[4 * S::QY * S::RP];
 
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
   const int i0 = 1 + (rxy % G.gx) * TX;
   const int j0 = 1 + (rxy / G.gx) * TY;
-  const int k0 = 1 + bz * G.kchunk;
+  const int k0 = 1 + bz * G.kchunk;              // odd: kchunk is even
   const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
-  const bool in = i0 >= 3 && i0 + TX + 1 <= L.nx && j0 >= 3 && j0 + TY + 1 <= L.ny;
-  if (in)
-    rb_fused_body<T, TX, TY, MODE, PF, true>(xo, xi, b, L, omega, i0, j0, k0, k1, sa, sb, sr);
-  else
-    rb_fused_body<T, TX, TY, MODE, PF, false>(xo, xi, b, L, omega, i0, j0, k0, k1, sa, sb, sr);
+  rb_fused_body<T, TX, TY, MODE, PF>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr);
 }
 
 template <typename T, int TX, int TY, int MODE, int PF>

@@ -30,50 +30,55 @@ struct MarchGrid {
 };
 
 // Stages x planes (4 slots) of a 64 x TY tile: x = i0-2 .. i0+65, y = j0-1 .. j0+TY.
+// issue() loads one plane into registers from clamped in-bounds addresses
+// with no branch around the loads; stage() zeroes what lies outside the
+// domain, adds the FP32 correction (MIXED) and writes LDS.  Keeping every
+// select away from the loads lets the compiler wait only for the loads a
+// plane needs instead of draining the ones in flight.
 template <typename T, int TY, bool MIXED>
 struct XStager {
   static constexpr int RX = 68, RY = TY + 2, XP = RX / 2;
   static constexpr int NXL = (RY * XP + MARCH_THREADS - 1) / MARCH_THREADS;
   using V2 = typename Vec2<T>::type;
   V2 pre[NXL];
+  float2 epre[MIXED ? NXL : 1];
+  int mq;
 
   __device__ __forceinline__ void issue(const T* __restrict__ x, const float* __restrict__ e, const Lvl<T>& L, int i0,
                                         int j0, int m) {
+    mq = m;
+    const int mc = min(max(m, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
-      int idx = threadIdx.x + q * MARCH_THREADS;
-      V2 v;
-      v.x = T(0);
-      v.y = T(0);
-      if (idx < RY * XP && m >= 1 && m <= L.nz) {
-        int ry = idx / XP, px = idx - ry * XP;
-        int gi = i0 - 2 + 2 * px, gj = j0 - 1 + ry;
-        if (gj >= 1 && gj <= L.ny) {
-          long q0 = lidx(L, gi, gj, m);
-          if (gi >= 1 && gi + 1 <= L.nx) {
-            v = *reinterpret_cast<const V2*>(x + q0);
-            if (MIXED) {
-              float2 ev = *reinterpret_cast<const float2*>(e + q0);
-              v.x = v.x + (T)ev.x;
-              v.y = v.y + (T)ev.y;
-            }
-          } else {
-            if (gi >= 1 && gi <= L.nx) v.x = MIXED ? x[q0] + (T)e[q0] : x[q0];
-            if (gi + 1 >= 1 && gi + 1 <= L.nx) v.y = MIXED ? x[q0 + 1] + (T)e[q0 + 1] : x[q0 + 1];
-          }
-        }
-      }
-      pre[q] = v;
+      int idx = min((int)threadIdx.x + q * MARCH_THREADS, RY * XP - 1);
+      int ry = idx / XP, px = idx - ry * XP;
+      int gi = i0 - 2 + 2 * px, gj = j0 - 1 + ry;      // gi is odd
+      // an odd gi <= nx keeps the 2-element load inside the padded row
+      int gic = (gi >= 1 && gi <= L.nx) ? gi : 1;
+      int gjc = min(max(gj, 1), L.ny);
+      long q0 = lidx(L, gic, gjc, mc);
+      pre[q] = *reinterpret_cast<const V2*>(x + q0);
+      if (MIXED) epre[q] = *reinterpret_cast<const float2*>(e + q0);
     }
   }
-  __device__ __forceinline__ void stage(T (*s)[RX]) {
+  __device__ __forceinline__ void stage(T (*s)[RX], const Lvl<T>& L, int i0, int j0) {
+    const bool vz = mq >= 1 && mq <= L.nz;
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
       int idx = threadIdx.x + q * MARCH_THREADS;
       if (idx < RY * XP) {
         int ry = idx / XP, px = idx - ry * XP;
-        s[ry][2 * px] = pre[q].x;
-        s[ry][2 * px + 1] = pre[q].y;
+        int gi = i0 - 2 + 2 * px, gj = j0 - 1 + ry;
+        bool vrow = vz && gj >= 1 && gj <= L.ny;
+        bool va = vrow && gi >= 1 && gi <= L.nx;
+        bool vb = vrow && gi + 1 >= 1 && gi + 1 <= L.nx;
+        T a = pre[q].x, b = pre[q].y;
+        if (MIXED) {
+          a = a + (T)epre[q].x;
+          b = b + (T)epre[q].y;
+        }
+        s[ry][2 * px] = va ? a : T(0);
+        s[ry][2 * px + 1] = vb ? b : T(0);
       }
     }
   }
@@ -132,31 +137,26 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
 
   St st;
   V2 bpre[NP];
+  // b of this thread's pairs, from clamped addresses (values of cells
+  // outside the domain are never used)
   auto issue_b = [&](int m) {
+    const int mc = min(max(m, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
-      V2 v;
-      v.x = T(0);
-      v.y = T(0);
-      if (act[q] && m >= 1 && m <= L.nz) {
-        int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
-        long q0 = lidx(L, gi, gj, m);
-        if (gi + 1 <= L.nx) v = *reinterpret_cast<const V2*>(b + q0);
-        else v.x = b[q0];
-      }
-      bpre[q] = v;
+      int gi = act[q] ? i0 + 2 * lp[q] : 1, gj = act[q] ? j0 + ly[q] : 1;
+      bpre[q] = *reinterpret_cast<const V2*>(b + lidx(L, gi, gj, mc));
     }
   };
 
   st.issue(x, e, L, i0, j0, k0 - 1);
-  st.stage(sx[(k0 - 1) & 3]);
+  st.stage(sx[(k0 - 1) & 3], L, i0, j0);
   st.issue(x, e, L, i0, j0, k0);
-  st.stage(sx[k0 & 3]);
+  st.stage(sx[k0 & 3], L, i0, j0);
   st.issue(x, e, L, i0, j0, k0 + 1);
   issue_b(k0);
   double acc = 0.0;
   for (int m = k0; m < k1; m++) {
-    st.stage(sx[(m + 1) & 3]);
+    st.stage(sx[(m + 1) & 3], L, i0, j0);
     lds_barrier();
     T(*s0)[RX] = sx[m & 3];
     T(*sm)[RX] = sx[(m - 1) & 3];
@@ -267,30 +267,25 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
 
   St st;
   V2 bpre[NP];
+  // b of this thread's pairs, from clamped addresses (values of cells
+  // outside the domain are never used)
   auto issue_b = [&](int m) {
+    const int mc = min(max(m, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
-      V2 v;
-      v.x = T(0);
-      v.y = T(0);
-      if (act[q] && m >= 1 && m <= L.nz) {
-        int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
-        long q0 = lidx(L, gi, gj, m);
-        if (gi + 1 <= L.nx) v = *reinterpret_cast<const V2*>(b + q0);
-        else v.x = b[q0];
-      }
-      bpre[q] = v;
+      int gi = act[q] ? i0 + 2 * lp[q] : 1, gj = act[q] ? j0 + ly[q] : 1;
+      bpre[q] = *reinterpret_cast<const V2*>(b + lidx(L, gi, gj, mc));
     }
   };
 
   st.issue(x, nullptr, L, i0, j0, k0 - 1);
-  st.stage(sx[(k0 - 1) & 3]);
+  st.stage(sx[(k0 - 1) & 3], L, i0, j0);
   st.issue(x, nullptr, L, i0, j0, k0);
-  st.stage(sx[k0 & 3]);
+  st.stage(sx[k0 & 3], L, i0, j0);
   st.issue(x, nullptr, L, i0, j0, k0 + 1);
   issue_b(k0);
   for (int m = k0; m < k1; m++) {
-    st.stage(sx[(m + 1) & 3]);
+    st.stage(sx[(m + 1) & 3], L, i0, j0);
     lds_barrier();
     T(*s0)[RX] = sx[m & 3];
     T(*sm)[RX] = sx[(m - 1) & 3];

@@ -37,8 +37,11 @@ struct Ops {
     G.gy = (d.ny + TY - 1) / TY;
     long nxy = (long)G.gx * G.gy;
     long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + 3) / 4));
+    if (nch < 8 && d.nz >= 64) nch = 8;   // one z-slab per XCD: tile rings hit the XCD's own L2
     if (nch >= 8) nch = (nch + 7) / 8 * 8;
-    G.kchunk = (int)((d.nz + nch - 1) / nch);
+    int kc = (int)((d.nz + nch - 1) / nch);
+    if (kc & 1) kc++;                      // even chunks: every chunk starts on an odd plane
+    G.kchunk = kc;
     G.nch = (d.nz + G.kchunk - 1) / G.kchunk;
     return G;
   }
@@ -61,7 +64,9 @@ struct Ops {
   // start (x taken as 0).  prefetch: planes of look-ahead of the global loads.
   int prefetch = 1;
   void fused_iter(DevLevel<T>& d, T omega, int mode) {
-    if (tile_y == 16) {
+    // tile height: 16 rows in FP64, 8 in FP32 unless set (measured fastest at 512^3)
+    int ty = tile_y ? tile_y : (sizeof(T) == 8 ? 16 : 8);
+    if (ty == 16) {
       if (prefetch == 2) fused_mode<16, 2>(d, omega, mode);
       else fused_mode<16, 1>(d, omega, mode);
     } else {
@@ -108,6 +113,7 @@ struct Ops {
     G.gy = (d.ny + TY - 1) / TY;
     long nxy = (long)G.gx * G.gy;
     long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + 3) / 4));
+    if (nch < 8 && d.nz >= 64) nch = 8;
     if (nch >= 8) nch = (nch + 7) / 8 * 8;
     int kc = (int)((d.nz + nch - 1) / nch);
     if (even && (kc & 1)) kc++;

@@ -369,9 +369,25 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   int ncyc = 0, nref = 0;
   double rho = 1.0;
   float om = (float)omega;
-  std::vector<int> cyc_it;
+  // Per-V-cycle reduction factor of the previous step.  A step that is
+  // predicted (with a factor-2 margin) to reach the tolerance in fewer than
+  // inner_cycles cycles runs only those: typically the last step, which needs
+  // a small reduction, while FP32 storage caps every step at about 1e-3 (so
+  // a capped step overstates the factor and the prediction stays safe).
+  double fcyc = 0.0;
   for (int s = 1; s <= c->opt.max_refine && ncyc < c->ncycle; s++) {
-    for (int q = 0; q < c->opt.inner_cycles; q++) {
+    int nin = c->opt.inner_cycles;
+    if (fcyc > 0.0 && fcyc < 1.0) {
+      double need = 0.5 * tol / rho, red = fcyc;
+      int k = 1;
+      while (k < nin && red > need) {
+        k++;
+        red *= fcyc;
+      }
+      nin = k;
+    }
+    double rho_prev = rho;
+    for (int q = 0; q < nin; q++) {
       int* slot = (ncyc < c->citers_cap) ? c->d_citers + ncyc : nullptr;
       smooth_level(c, op, g, cg, 0, nu, om, q == 0, false);
       op.residual_restrict(f32, g.lv[1], f32.x, f32.b);
@@ -387,6 +403,7 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     nref = s;
     if (c->opt.verbose) print_cycle(s, hs[0], res0);
     if (rho < tol) break;
+    if (rho > 0.0 && rho_prev > 0.0) fcyc = std::pow(rho / rho_prev, 1.0 / nin);
   }
   if (ncyc > 0) {
     int m = std::min(ncyc, c->citers_cap);
@@ -552,7 +569,7 @@ int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double
   double n = 0, es = precision == 1 ? 4.0 : 8.0, per = 0;
   const char* name = "";
   bool fused = c->opt.fused_smoother != 0;
-  int ty = c->opt.tile_y == 16 ? 16 : 8;
+  int ty_opt = c->opt.tile_y;
   // fused: one red+black iteration reads x and b and writes x once (3 n
   // elements); colour sweep: reads x and b, writes half of x (2 n)
   auto body = [&](auto& op, auto& d) {
@@ -565,6 +582,7 @@ int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double
   };
   if (precision == 1) {
     Ops<float> op{c->stream};
+    int ty = ty_opt ? ty_opt : 8;
     op.tile_y = ty;
     op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
     DevLevel<float>& d = c->h32.lv[0];
@@ -578,6 +596,7 @@ int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double
                                         : (ty == 16 ? "k_rb_fused<float, 128, 16, 0, 1>" : "k_rb_fused<float, 128, 8, 0, 1>");
   } else {
     Ops<double> op{c->stream};
+    int ty = ty_opt ? ty_opt : 16;
     op.tile_y = ty;
     op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
     DevLevel<double>& d = c->h64.lv[0];
