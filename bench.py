@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--lz", type=float, default=1.0, help="domain length in z (2.0: hx:hy:hz = 1:1:2)")
     p.add_argument("--inner", type=int, default=0, help="FP32 V-cycles per refinement step (0: default)")
     p.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the two rocprofv3 counter passes that measure the smoother's HBM traffic")
     p.add_argument("--smoother-reps", type=int, default=20)
     p.add_argument("--adjoint", action="store_true", help="run the adjoint inverse-source workload instead")
     p.add_argument("--adjoint-n", type=int, default=128)
@@ -77,6 +79,56 @@ def kernels_only(a):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measure_traffic(a, kernel):
+    """HBM bytes per launch of `kernel` from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch) over this script's
+    --kernels-only mode, each scaled by what the same counter reads for the
+    1 GiB device copy that mode also runs (copy bytes / counter).  Each pass
+    runs as a child process with its own time limit; None if a pass fails."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="pepsmg_pmc_")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
+               os.path.abspath(__file__), "--kernels-only", "--smoother-reps", "2", "--n", str(a.n), "--lz",
+               str(a.lz)]
+        log(f"[bench] counter pass {counter}")
+        proc = subprocess.Popen(cmd, cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                start_new_session=True)
+        try:
+            rc = proc.wait(timeout=150)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.wait()
+            return None, f"{counter} pass timed out"
+        if rc != 0:
+            return None, f"{counter} pass exited with {rc}"
+        path = os.path.join(d, "pmc_counter_collection.csv")
+        if not os.path.exists(path):
+            return None, f"{counter} pass wrote no counters"
+        vals, copy = [], []
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"]
+            if kernel.replace(" ", "") in name.replace(" ", ""):
+                vals.append(float(r["Counter_Value"]))
+            elif "copyBuffer" in name and float(r["Counter_Value"]) > 1e5:
+                copy.append(float(r["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if not vals or not copy:
+            return None, f"{counter}: kernel or calibration copy not found"
+        scale = (1 << 30) / 1024.0 / (sum(copy) / len(copy))   # KiB the copy moves / KiB counted
+        out[counter] = sum(vals) / len(vals) * 1024.0 * scale
+        out[counter + "_scale"] = scale
+    return out, None
 
 
 def adjoint_workload(a):
@@ -263,6 +315,16 @@ def main():
                                                 C.byref(nbytes32), C.byref(kname32)))
     achieved32 = nbytes32.value / (avg32.value * 1e-3) / 1e9
 
+    traffic, traffic_note, tr = None, "not measured (--no-traffic)", None
+    if not a.no_traffic:
+        tr, err = measure_traffic(a, kname.value.decode())
+        if tr:
+            traffic = tr["FETCH_SIZE"] + tr["WRITE_SIZE"]
+            traffic_note = (f"rocprofv3 FETCH_SIZE x {tr['FETCH_SIZE_scale']:.3f} + WRITE_SIZE x "
+                            f"{tr['WRITE_SIZE_scale']:.3f} (scales from a 1 GiB device copy)")
+        else:
+            traffic_note = err
+
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu_baseline: oracle, 2 V-cycles")
@@ -300,7 +362,10 @@ def main():
         "rho_mixed": rmx.rho,
         "mixed_vs_fp64_error": {"max_rel": err_max, "l2_rel": err_l2},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic) if traffic else None, "traffic_note": traffic_note,
+                     "traffic_read": round(tr["FETCH_SIZE"]) if tr else None,
+                     "traffic_write": round(tr["WRITE_SIZE"]) if tr else None,
                      "kernel": kname.value.decode(), "avg_ms": avg.value, "bytes_per_launch": nbytes.value},
         "roofline_fp32_smoother": {"achieved": round(achieved32, 1), "frac": round(achieved32 / HBM_PEAK_GBS, 4),
                                    "kernel": kname32.value.decode(), "avg_ms": avg32.value,

@@ -91,14 +91,16 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
       }
     }
   };
-  V2 xpre[NP];
-  auto issue_f = [&](int f) {
+  // fine planes are loaded two planes ahead: xpre[h] holds the next fine
+  // plane of parity h (h = 0: odd, 1: even)
+  V2 xpre[2][NP];
+  auto issue_f = [&](V2* dst, int f) {
     const int fc = min(max(f, 1), F.nz);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
       // an odd column <= nx keeps the 2-element load inside the padded row
       int gi = act[q] ? ia : 1, gj = act[q] ? j0 + ly[q] : 1;
-      xpre[q] = *reinterpret_cast<const V2*>(xf + lidx(F, gi, gj, fc));
+      dst[q] = *reinterpret_cast<const V2*>(xf + lidx(F, gi, gj, fc));
     }
   };
 
@@ -109,17 +111,19 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
   issue_c(kc0 + 1);
   stage_c(kc0 + 1);
   issue_c(kc0 + 2);
-  issue_f(2 * kc0 - 1);
-  for (int kc = kc0; kc < kc1; kc++) {
+  issue_f(xpre[0], 2 * kc0 - 1);
+  issue_f(xpre[1], 2 * kc0);
+  // coarse-plane step, run twice per iteration so the compiler's wait
+  // counts across the back edge stay exact
+  auto step = [&](int kc) {
     lds_barrier();
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      int f = 2 * kc - 1 + h;                        // h = 0: odd fine plane, 1: even
-      if (f > F.nz) break;
+      int f = 2 * kc - 1 + h;                        // h = 0: odd fine plane, 1: even (F.nz = 2 Cc.nz)
       V2 xv[NP];
 #pragma unroll
-      for (int q = 0; q < NP; q++) xv[q] = xpre[q];
-      issue_f(f + 1);
+      for (int q = 0; q < NP; q++) xv[q] = xpre[h][q];
+      issue_f(xpre[h], f + 2);
       T wsz = (h == 0) ? ldc(W.pz_, f) : ldc(W.nz_, f);
       T wnz = (h == 0) ? ldc(W.pz_, f + 1) : ldc(W.nz_, f - 1);
       T(*P)[CX] = sc[kc & 3];
@@ -153,7 +157,13 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
     }
     stage_c(kc + 2);
     issue_c(kc + 3);
+  };
+  int kc = kc0;
+  for (; kc + 1 < kc1; kc += 2) {
+    step(kc);
+    step(kc + 1);
   }
+  if (kc < kc1) step(kc);
 }
 
 template <typename T>

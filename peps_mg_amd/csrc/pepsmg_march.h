@@ -155,7 +155,10 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
   st.issue(x, e, L, i0, j0, k0 + 1);
   issue_b(k0);
   double acc = 0.0;
-  for (int m = k0; m < k1; m++) {
+  // plane step; the loop below runs it twice per iteration so that the
+  // loads in flight across the back edge sit in the same registers as the
+  // prologue's and the compiler's wait counts stay exact
+  auto body = [&](int m) {
     st.stage(sx[(m + 1) & 3], L, i0, j0);
     lds_barrier();
     T(*s0)[RX] = sx[m & 3];
@@ -204,7 +207,13 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
         }
       }
     }
+  };
+  int m = k0;
+  for (; m + 1 < k1; m += 2) {
+    body(m);
+    body(m + 1);
   }
+  if (m < k1) body(m);
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[bid] = s;
 }
@@ -284,7 +293,9 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
   st.stage(sx[k0 & 3], L, i0, j0);
   st.issue(x, nullptr, L, i0, j0, k0 + 1);
   issue_b(k0);
-  for (int m = k0; m < k1; m++) {
+  // plane step, run twice per iteration (chunks hold an even number of
+  // planes) so the compiler's wait counts across the back edge stay exact
+  auto step = [&](int m) {
     st.stage(sx[(m + 1) & 3], L, i0, j0);
     lds_barrier();
     T(*s0)[RX] = sx[m & 3];
@@ -325,6 +336,10 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
         bc[lidx(Cc, ic, jc, kc)] = s / vol_c;
       }
     }
+  };
+  for (int m = k0; m < k1; m += 2) {
+    step(m);
+    step(m + 1);
   }
 }
 
