@@ -175,6 +175,38 @@ int pepsmg_fill_rhs_host(int32_t nx, int32_t ny, int32_t nz, double *out, uint64
                          double scale_rand, double scale_sin, const double *xg, const double *yg,
                          const double *zg);
 
+/* ---- Batched solves (no counterpart in the reference, which solves one
+ * grid per call).  A batch holds nprob independent problems, problem p with
+ * its own grid (grids[p]), number of coarse levels (nlevel[p], the
+ * multigrid_create rule of src/multigrid.f90:307-314 with aggregation off)
+ * and nrhs[p] right-hand sides; each (problem, right-hand side) pair is an
+ * item, numbered problem by problem.  Every V-cycle operation runs as one
+ * launch over all items and a solve replays one captured hipGraph per
+ * V-cycle (FP64) or refinement step (precision 1) until every item has
+ * rho < tolerance.  Options as for a single solve; the batch needs the
+ * red/black smoother, the fused smoother and the persistent coarse kernel
+ * (coarse_switch > 0, and each item must have a coarse level that fits it). */
+typedef struct pepsmg_batch pepsmg_batch;
+
+int pepsmg_batch_create(int32_t nprob, const pepsmg_grid *grids, const int32_t *nlevel, const int32_t *nrhs,
+                        int32_t ncycle, pepsmg_batch **out);
+
+int pepsmg_batch_set_options(pepsmg_batch *batch, const pepsmg_options *opt);
+
+/* Number of items (sum of nrhs). */
+int32_t pepsmg_batch_size(const pepsmg_batch *batch);
+
+/* Solves every item from its initial guess.  d_sol[i] / d_rhs[i]: device
+ * arrays of item i in the padded layout (ghosts zero), overwritten /
+ * read.  use_graph 1 replays the captured graph, 0 launches the same
+ * sequence directly.  stats: NULL or an array of batch_size entries
+ * (history as for a single solve; device_ms is the time of the whole batch). */
+int pepsmg_batch_solve_device(pepsmg_batch *batch, double *const *d_sol, const double *const *d_rhs,
+                              int32_t maxiteration, double tolerance, double omega_sor, int32_t use_graph,
+                              void *stream, pepsmg_stats *stats);
+
+void pepsmg_batch_destroy(pepsmg_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,12 @@ SIGNATURES = {
                                          C.c_double, _dp, _dp, _dp, C.c_void_p]),
     "pepsmg_fill_rhs_host": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _dp, C.c_uint64, C.c_double, C.c_double,
                                        _dp, _dp, _dp]),
+    "pepsmg_batch_create": (C.c_int, [C.c_int32, C.POINTER(Grid), _ip, _ip, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pepsmg_batch_set_options": (C.c_int, [C.c_void_p, C.POINTER(Options)]),
+    "pepsmg_batch_size": (C.c_int32, [C.c_void_p]),
+    "pepsmg_batch_solve_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
+                                            C.c_double, C.c_double, C.c_int32, C.c_void_p, C.POINTER(Stats)]),
+    "pepsmg_batch_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None

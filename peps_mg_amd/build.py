@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libpepsmg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PEPSMG_ARCH", "gfx950")
 
-SOURCES = ["pepsmg_core.hip", "pepsmg_solve.hip"]
+SOURCES = ["pepsmg_core.hip", "pepsmg_solve.hip", "pepsmg_batch.hip"]
 
 # -ffp-contract=off: no fused multiply-add anywhere, so FP64 results are
 # the reference's bit for bit.  -packed-fp32-ops: no packed FP32 VALU ops.

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "pepsmg_coarse.h"
 #include "pepsmg_fused.h"
@@ -200,5 +201,57 @@ struct Ops {
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
   }
 };
+
+
+// Arguments of the persistent coarse kernel for levels s..L of hierarchy h:
+// compact LDS layout of every level's x and b (16-byte aligned), the
+// coarsest level in FP64.  Returns false if it does not fit in LDS.
+template <typename T>
+inline bool pc_build(int L, Hier<double>& h64, size_t lds_max, Hier<T>& h, int s, int nu, T omega, double tol,
+                     int* slot, PCArgs<T>& A, size_t& lds) {
+  int nlev = L - s + 1;
+  if (s < 1 || nlev > PC_MAXLEV) return false;
+  std::memset(&A, 0, sizeof A);
+  A.nlev = nlev;
+  A.nu = nu;
+  A.omega = omega;
+  A.tol = tol;
+  A.maxit = 1000;
+  A.iters_slot = slot;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = (off + bytes + 15) / 16 * 16;
+    return o;
+  };
+  for (int q = 0; q < nlev; q++) {
+    DevLevel<T>& d = h.lv[s + q];
+    Lvl<T> l = d.L();
+    l.px = d.nx;
+    l.pxy = (long)d.nx * d.ny;
+    A.lv[q] = l;
+    if (q < nlev - 1) {
+      size_t n = (size_t)d.cells();
+      A.xoff[q] = (int)(take(n * sizeof(T)) / sizeof(T));
+      A.boff[q] = (int)(take(n * sizeof(T)) / sizeof(T));
+      A.rw[q] = d.RW();
+      A.pw[q] = d.PW();
+    }
+  }
+  DevLevel<double>& cd = h64.lv[L];
+  Lvl<double> lc = cd.L();
+  lc.px = cd.nx;
+  lc.pxy = (long)cd.nx * cd.ny;
+  A.lc = lc;
+  size_t nc = (size_t)cd.cells();
+  A.xcoff = (int)(take(nc * 8) / 8);
+  A.bcoff = (int)(take(nc * 8) / 8);
+  A.xctoff = (int)(take(nc * sizeof(T)) / sizeof(T));
+  A.gl = h.lv[s].L();
+  A.gxs = h.lv[s].x;
+  A.gbs = h.lv[s].b;
+  lds = off;
+  return off <= lds_max;
+}
 
 }  // namespace pepsmg

@@ -164,58 +164,6 @@ static void coarse_solve64(pepsmg_ctx* c, DevLevel<double>& d, double omega, dou
                      omega, tol, 1000, (int*)nullptr, slot);
 }
 
-// Arguments of the persistent coarse kernel for levels s..L of hierarchy h:
-// compact LDS layout of every level's x and b (16-byte aligned), the
-// coarsest level in FP64.  Returns false if it does not fit in LDS.
-template <typename T>
-static bool pc_build(pepsmg_ctx* c, Hier<T>& h, int s, int nu, T omega, double tol, int* slot, PCArgs<T>& A,
-                     size_t& lds) {
-  int L = c->nlevel;
-  int nlev = L - s + 1;
-  if (s < 1 || nlev > PC_MAXLEV) return false;
-  std::memset(&A, 0, sizeof A);
-  A.nlev = nlev;
-  A.nu = nu;
-  A.omega = omega;
-  A.tol = tol;
-  A.maxit = 1000;
-  A.iters_slot = slot;
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off = (off + bytes + 15) / 16 * 16;
-    return o;
-  };
-  for (int q = 0; q < nlev; q++) {
-    DevLevel<T>& d = h.lv[s + q];
-    Lvl<T> l = d.L();
-    l.px = d.nx;
-    l.pxy = (long)d.nx * d.ny;
-    A.lv[q] = l;
-    if (q < nlev - 1) {
-      size_t n = (size_t)d.cells();
-      A.xoff[q] = (int)(take(n * sizeof(T)) / sizeof(T));
-      A.boff[q] = (int)(take(n * sizeof(T)) / sizeof(T));
-      A.rw[q] = d.RW();
-      A.pw[q] = d.PW();
-    }
-  }
-  DevLevel<double>& cd = c->h64.lv[L];
-  Lvl<double> lc = cd.L();
-  lc.px = cd.nx;
-  lc.pxy = (long)cd.nx * cd.ny;
-  A.lc = lc;
-  size_t nc = (size_t)cd.cells();
-  A.xcoff = (int)(take(nc * 8) / 8);
-  A.bcoff = (int)(take(nc * 8) / 8);
-  A.xctoff = (int)(take(nc * sizeof(T)) / sizeof(T));
-  A.gl = h.lv[s].L();
-  A.gxs = h.lv[s].x;
-  A.gbs = h.lv[s].b;
-  lds = off;
-  return off <= c->lds_max;
-}
-
 template <typename T>
 static void pc_launch(pepsmg_ctx* c, const PCArgs<T>& A, size_t lds) {
   static bool attr = false;
@@ -256,7 +204,8 @@ static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>*
   size_t lds = 0;
   if (c->opt.smoother == 0 && c->opt.coarse_switch > 0) {
     for (int l = 1; l <= L; l++) {
-      if (h.lv[l].cells() <= c->opt.coarse_switch && pc_build(c, h, l, nu, omega, tol, slot, A, lds)) {
+      if (h.lv[l].cells() <= c->opt.coarse_switch &&
+          pc_build(c->nlevel, c->h64, c->lds_max, h, l, nu, omega, tol, slot, A, lds)) {
         s = l;
         break;
       }
