@@ -43,6 +43,7 @@ def parse():
     p.add_argument("--smoother-reps", type=int, default=20)
     p.add_argument("--adjoint", action="store_true", help="run the adjoint inverse-source workload instead")
     p.add_argument("--adjoint-n", type=int, default=128)
+    p.add_argument("--no-tune", action="store_true", help="batched workload: skip the autotuner")
     p.add_argument("--batch", action="store_true",
                    help="run the batched workload (48 cubes of mixed sizes, 8 with 4 right-hand sides) instead")
     p.add_argument("--kernels-only", action="store_true",
@@ -256,8 +257,13 @@ def batch_workload(a):
         return (time.perf_counter() - t0) / steps
 
     out = {}
+    tuned = {}
     for prec, tag in ((1, "mixed"), (0, "fp64")):
         B = BatchSolver(sdms, levels, nrhs, ncycle=100, precision=prec)
+        if not a.no_tune:
+            from peps_mg_amd.autotune import tune_batch
+            tuned[tag] = tune_batch(B, sols, rhss, 2, 1e-10, 1.0)
+            log(f"[bench] autotuned {tag}: {tuned[tag]}")
         res = []
 
         def run():
@@ -293,7 +299,7 @@ def batch_workload(a):
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "48 unit cubes, edges from {32,48,64,96,128,192} (seed 20210501), 8 with 4 RHS, "
                                "U[-1,1] RHS, V(2,2) RBGS, tol 1e-10", "solves": len(rhss), "unknowns": unknowns},
-        "batch_mixed": out["batch_mixed"], "batch_fp64": out["batch_fp64"],
+        "batch_mixed": out["batch_mixed"], "batch_fp64": out["batch_fp64"], "autotuned": tuned,
         "looped_single_fp64_s": round(t_loop, 5), "looped_single_fp64_unknowns_per_s": round(unknowns / t_loop, 1),
         "speedup_batch_mixed_vs_looped_fp64": round(t_loop / out["batch_mixed"]["s"], 3)}))
 

@@ -65,6 +65,7 @@ typedef struct pepsmg_options {
                               plane ahead; 2: the same with loads two planes ahead; 0: two colour sweeps */
   int32_t tile_y;          /* fused smoother tile height 8 or 16 cells (0: 16 in FP64, 8 in FP32) */
   int32_t verbose;         /* 1: print the reference's [MG] cycle lines */
+  int32_t tile_x;          /* FP32 fused smoother tile width 64 or 128 cells (0: 128); FP64 uses 64 */
 } pepsmg_options;
 
 /* Per-solve statistics (the reference only prints these, src/multigrid.f90:1972). */
@@ -195,6 +196,11 @@ int pepsmg_batch_set_options(pepsmg_batch *batch, const pepsmg_options *opt);
 
 /* Number of items (sum of nrhs). */
 int32_t pepsmg_batch_size(const pepsmg_batch *batch);
+
+/* Fused-smoother tile shape per problem (arrays of nprob; 0 or NULL: the
+ * options' tile_x / tile_y).  Items of different shapes run in separate
+ * launches; the autotuner picks the shape per grid size. */
+int pepsmg_batch_set_tiles(pepsmg_batch *batch, const int32_t *tile_x, const int32_t *tile_y);
 
 /* Solves every item from its initial guess.  d_sol[i] / d_rhs[i]: device
  * arrays of item i in the padded layout (ghosts zero), overwritten /

@@ -32,7 +32,7 @@ class Grid(C.Structure):
 class Options(C.Structure):
     _fields_ = [("smoother", C.c_int32), ("precision", C.c_int32), ("coarse_switch", C.c_int32),
                 ("inner_cycles", C.c_int32), ("max_refine", C.c_int32), ("fused_smoother", C.c_int32),
-                ("tile_y", C.c_int32), ("verbose", C.c_int32)]
+                ("tile_y", C.c_int32), ("verbose", C.c_int32), ("tile_x", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -70,6 +70,7 @@ SIGNATURES = {
     "pepsmg_batch_create": (C.c_int, [C.c_int32, C.POINTER(Grid), _ip, _ip, C.c_int32, C.POINTER(C.c_void_p)]),
     "pepsmg_batch_set_options": (C.c_int, [C.c_void_p, C.POINTER(Options)]),
     "pepsmg_batch_size": (C.c_int32, [C.c_void_p]),
+    "pepsmg_batch_set_tiles": (C.c_int, [C.c_void_p, _ip, _ip]),
     "pepsmg_batch_solve_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
                                             C.c_double, C.c_double, C.c_int32, C.c_void_p, C.POINTER(Stats)]),
     "pepsmg_batch_destroy": (None, [C.c_void_p]),

@@ -23,6 +23,7 @@ class BatchSolver:
         if not (len(sdms) == len(nlevels) == len(nrhs)) or not sdms:
             raise _lib.PepsmgError(_lib.ERR_ARG, "sdms, nlevels and nrhs must have the same non-zero length")
         self.sdms = list(sdms)
+        self.nlevels = [int(x) for x in nlevels]
         self.grids = (_lib.Grid * len(sdms))(*[grid_descriptor(s) for s in sdms])
         self.nrhs = [int(r) for r in nrhs]
         self.ncycle = int(ncycle)
@@ -48,6 +49,14 @@ class BatchSolver:
                 raise KeyError(f"unknown option {k}")
             setattr(self.options, k, int(v))
         _lib.check(_lib.lib().pepsmg_batch_set_options(self.h, C.byref(self.options)))
+
+    def set_tiles(self, tile_x=None, tile_y=None):
+        """Fused-smoother tile shape per problem (lists of nprob; None or 0:
+        the options' values)."""
+        n = len(self.sdms)
+        ax = (C.c_int32 * n)(*tile_x) if tile_x is not None else None
+        ay = (C.c_int32 * n)(*tile_y) if tile_y is not None else None
+        _lib.check(_lib.lib().pepsmg_batch_set_tiles(self.h, ax, ay))
 
     def solve_device(self, sols, rhss, maxiteration, tolerance, omega_sor, use_graph=True, stream=None):
         """sols / rhss: one padded float64 device array per item (integer

@@ -35,9 +35,22 @@ def _run(cmd, cwd=None):
 
 
 def build_library(verbose=True):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    """Compiles the translation units in parallel (one hipcc each), then links."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    procs, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = [HIPCC] + compile_flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), cmd))
+        objs.append(obj)
+    failed = [cmd for p, cmd in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + srcs + ["-o", tmp])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-fPIC", "-shared"] + objs + ["-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

@@ -192,6 +192,7 @@ struct BItem {
   int prob = 0;
   int nlevel = 0;
   int s = 0;              // first level of the persistent coarse kernel
+  int tile_x = 0, tile_y = 0;   // fused smoother tile of this item's problem (0: options)
   Hier<double> h64;
   Hier<float> h32;
   double* x_after_init = nullptr;   // fine FP64 x buffer holding the iterate after the initial update
@@ -290,63 +291,60 @@ namespace pepsmg {
 // fine-level residual-march grid (same decomposition as the single-grid solver)
 static MarchGrid resid_grid(const DevLevel<double>& f) { return Ops<double>::march_grid(f, 8, false); }
 
-template <typename T>
-static int tile_of(const pepsmg_options& o) {
-  return o.tile_y ? o.tile_y : (sizeof(T) == 8 ? 16 : 8);
+// Record nu fused iterations on level l of every item that has it (per-level
+// levels only), swapping each item's x / x2 as the launches do.  Items are
+// grouped by fused-smoother tile shape, one launch per group.  zero_first:
+// the first iteration starts from x = 0.
+template <typename T, int TX, int TY>
+static void rec_smooth_group(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, const std::vector<int>& members,
+                             int l, bool zero, T omega) {
+  std::vector<FusedItem<T>> tab;
+  int nb = 0;
+  for (int i : members) {
+    DevLevel<T>& d = H[i]->lv[l];
+    FusedItem<T> e;
+    e.block0 = nb;
+    e.item = i;
+    e.xo = d.x2;
+    e.xi = d.x;
+    e.b = d.b;
+    e.L = d.L();
+    e.G = Ops<T>::fused_grid(d, TX, TY);
+    nb += e.G.gx * e.G.gy * e.G.nch;
+    tab.push_back(e);
+    std::swap(d.x, d.x2);
+  }
+  size_t off = S.put(tab);
+  int n = (int)tab.size();
+  int* act = B->d_active;
+  S.ops.push_back([=](hipStream_t st, const char* dev) {
+    const FusedItem<T>* p = (const FusedItem<T>*)(dev + off);
+    if (zero) hipLaunchKernelGGL((kb_rb_fused<T, TX, TY, 1, 1>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
+    else hipLaunchKernelGGL((kb_rb_fused<T, TX, TY, 0, 1>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
+  });
 }
 
-// Record nu fused iterations on level l of every item that has it (per-level
-// levels only), swapping each item's x / x2 as the launches do.  zero_first:
-// the first iteration starts from x = 0.
 template <typename T>
 static void rec_smooth(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int l, int nu, T omega,
                        bool zero_first) {
-  constexpr int FTX = Ops<T>::FTX;
-  const int ty = tile_of<T>(B->opt);
-  const int pf = B->opt.fused_smoother == 2 ? 2 : 1;
+  // tile shape per item: width 64 (FP64 always) or 128 (FP32 default), height 8 or 16
+  std::vector<int> g[2][2];
+  for (int i = 0; i < B->nitems; i++) {
+    if (l >= B->items[i].s) continue;
+    int tx = B->items[i].tile_x ? B->items[i].tile_x : B->opt.tile_x;
+    int ty = B->items[i].tile_y ? B->items[i].tile_y : B->opt.tile_y;
+    if (!ty) ty = sizeof(T) == 8 ? 16 : 8;
+    int wx = (sizeof(T) == 4 && tx != 64) ? 1 : 0;
+    g[wx][ty == 16 ? 1 : 0].push_back(i);
+  }
   for (int it = 0; it < nu; it++) {
-    std::vector<FusedItem<T>> tab;
-    int nb = 0;
-    for (int i = 0; i < B->nitems; i++) {
-      if (l >= B->items[i].s) continue;
-      DevLevel<T>& d = H[i]->lv[l];
-      FusedItem<T> e;
-      e.block0 = nb;
-      e.item = i;
-      e.xo = d.x2;
-      e.xi = d.x;
-      e.b = d.b;
-      e.L = d.L();
-      e.G = Ops<T>::fused_grid(d, FTX, ty);
-      nb += e.G.gx * e.G.gy * e.G.nch;
-      tab.push_back(e);
-      std::swap(d.x, d.x2);
-    }
-    if (tab.empty()) return;
-    size_t off = S.put(tab);
-    int n = (int)tab.size();
     bool z = zero_first && it == 0;
-    int* act = B->d_active;
-    S.ops.push_back([=](hipStream_t st, const char* dev) {
-      const FusedItem<T>* p = (const FusedItem<T>*)(dev + off);
-      if (ty == 16) {
-        if (pf == 2) {
-          if (z) hipLaunchKernelGGL((kb_rb_fused<T, FTX, 16, 1, 2>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-          else hipLaunchKernelGGL((kb_rb_fused<T, FTX, 16, 0, 2>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-        } else {
-          if (z) hipLaunchKernelGGL((kb_rb_fused<T, FTX, 16, 1, 1>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-          else hipLaunchKernelGGL((kb_rb_fused<T, FTX, 16, 0, 1>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-        }
-      } else {
-        if (pf == 2) {
-          if (z) hipLaunchKernelGGL((kb_rb_fused<T, FTX, 8, 1, 2>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-          else hipLaunchKernelGGL((kb_rb_fused<T, FTX, 8, 0, 2>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-        } else {
-          if (z) hipLaunchKernelGGL((kb_rb_fused<T, FTX, 8, 1, 1>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-          else hipLaunchKernelGGL((kb_rb_fused<T, FTX, 8, 0, 1>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-        }
-      }
-    });
+    if (!g[0][0].empty()) rec_smooth_group<T, 64, 8>(B, S, H, g[0][0], l, z, omega);
+    if (!g[0][1].empty()) rec_smooth_group<T, 64, 16>(B, S, H, g[0][1], l, z, omega);
+    if (sizeof(T) == 4) {
+      if (!g[1][0].empty()) rec_smooth_group<T, Ops<T>::FTX, 8>(B, S, H, g[1][0], l, z, omega);
+      if (!g[1][1].empty()) rec_smooth_group<T, Ops<T>::FTX, 16>(B, S, H, g[1][1], l, z, omega);
+    }
   }
 }
 
@@ -593,6 +591,7 @@ int pepsmg_batch_set_options(pepsmg_batch* B, const pepsmg_options* opt) {
   if (opt->fused_smoother < 1 || opt->fused_smoother > 2)
     return fail(PEPSMG_ERR_ARG, "batched solves use the fused smoother (fused_smoother = 1 or 2)");
   if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16) return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8 or 16");
+  if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
   if (opt->inner_cycles < 1 || opt->max_refine < 1) return fail(PEPSMG_ERR_ARG, "inner_cycles/max_refine must be >= 1");
   if (opt->coarse_switch < 1) return fail(PEPSMG_ERR_ARG, "batched solves need the persistent coarse kernel (coarse_switch > 0)");
   B->opt = *opt;
@@ -608,6 +607,21 @@ int pepsmg_batch_set_options(pepsmg_batch* B, const pepsmg_options* opt) {
 }
 
 int32_t pepsmg_batch_size(const pepsmg_batch* B) { return B ? B->nitems : 0; }
+
+int pepsmg_batch_set_tiles(pepsmg_batch* B, const int32_t* tile_x, const int32_t* tile_y) {
+  if (!B) return fail(PEPSMG_ERR_ARG, "NULL argument");
+  for (int p = 0; p < B->nprob; p++) {
+    int tx = tile_x ? tile_x[p] : 0, ty = tile_y ? tile_y[p] : 0;
+    if (tx != 0 && tx != 64 && tx != 128) return fail(PEPSMG_ERR_ARG, "problem %d: tile_x must be 0, 64 or 128", p);
+    if (ty != 0 && ty != 8 && ty != 16) return fail(PEPSMG_ERR_ARG, "problem %d: tile_y must be 0, 8 or 16", p);
+  }
+  for (auto& it : B->items) {
+    it.tile_x = tile_x ? tile_x[it.prob] : 0;
+    it.tile_y = tile_y ? tile_y[it.prob] : 0;
+  }
+  B->drop_schedule();
+  return PEPSMG_OK;
+}
 
 void pepsmg_batch_destroy(pepsmg_batch* B) {
   if (!B) return;

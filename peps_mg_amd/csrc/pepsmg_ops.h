@@ -23,7 +23,7 @@ static inline dim3 grid3(int nx, int ny, int nz, int bx, int by) {
 template <typename T>
 struct Ops {
   hipStream_t s;
-  int tile_y = 8;
+  int tile_y = 0;
 
   // Block decomposition of the fused smoother: 64 x TY tiles, z-chunks of
   // at least 4 planes, about 2048 blocks, chunk count a multiple of 8 when
@@ -47,33 +47,44 @@ struct Ops {
     return G;
   }
 
-  template <int TY, int MODE, int PF>
+  template <int TX, int TY, int MODE, int PF>
   void fused_launch(DevLevel<T>& d, T omega) {
-    FusedGrid G = fused_grid(d, FTX, TY);
+    FusedGrid G = fused_grid(d, TX, TY);
     unsigned nb = (unsigned)(G.gx * G.gy * G.nch);
-    hipLaunchKernelGGL((k_rb_fused<T, FTX, TY, MODE, PF>), dim3(nb), dim3(FUSED_THREADS), 0, s, d.x2, d.x, d.b,
+    hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(FUSED_THREADS), 0, s, d.x2, d.x, d.b,
                        d.L(), omega, G);
   }
 
-  template <int TY, int PF>
+  template <int TX, int TY, int PF>
   void fused_mode(DevLevel<T>& d, T omega, int mode) {
-    if (mode == 1) fused_launch<TY, 1, PF>(d, omega);
-    else fused_launch<TY, 0, PF>(d, omega);
+    if (mode == 1) fused_launch<TX, TY, 1, PF>(d, omega);
+    else fused_launch<TX, TY, 0, PF>(d, omega);
   }
+
+  template <int TX>
+  void fused_tx(DevLevel<T>& d, T omega, int mode, int ty) {
+    if (ty == 16) {
+      if (prefetch == 2) fused_mode<TX, 16, 2>(d, omega, mode);
+      else fused_mode<TX, 16, 1>(d, omega, mode);
+    } else {
+      if (prefetch == 2) fused_mode<TX, 8, 2>(d, omega, mode);
+      else fused_mode<TX, 8, 1>(d, omega, mode);
+    }
+  }
+
+  // tile sizes in use: height 16 rows in FP64 and 8 in FP32, width 64 cells
+  // in FP64 and 128 in FP32 (the same bytes per row) unless set
+  int tile_x = 0;
+  int tx_used() const { return (sizeof(T) == 4 && tile_x == 64) ? 64 : FTX; }
+  int ty_used() const { return tile_y ? tile_y : (sizeof(T) == 8 ? 16 : 8); }
 
   // One fused red+black iteration x -> x2, then swap.  mode 0 plain, 1 zero
   // start (x taken as 0).  prefetch: planes of look-ahead of the global loads.
   int prefetch = 1;
   void fused_iter(DevLevel<T>& d, T omega, int mode) {
-    // tile height: 16 rows in FP64, 8 in FP32 unless set (measured fastest at 512^3)
-    int ty = tile_y ? tile_y : (sizeof(T) == 8 ? 16 : 8);
-    if (ty == 16) {
-      if (prefetch == 2) fused_mode<16, 2>(d, omega, mode);
-      else fused_mode<16, 1>(d, omega, mode);
-    } else {
-      if (prefetch == 2) fused_mode<8, 2>(d, omega, mode);
-      else fused_mode<8, 1>(d, omega, mode);
-    }
+    int ty = ty_used();
+    if (sizeof(T) == 4 && tx_used() == 64) fused_tx<64>(d, omega, mode, ty);
+    else fused_tx<FTX>(d, omega, mode, ty);
     std::swap(d.x, d.x2);
   }
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pepsmg.h"
@@ -240,6 +241,7 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
   Ops<double> op{c->stream};
   op.tile_y = c->opt.tile_y;
   op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
+  op.tile_x = c->opt.tile_x;
   CgBufs<double>* cg = nullptr;
   if (c->opt.smoother == 1) {
     cg = &c->cg64;
@@ -290,6 +292,7 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   Ops<float> op{c->stream};
   op.tile_y = c->opt.tile_y;
   op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
+  op.tile_x = c->opt.tile_x;
   CgBufs<float>* cg = nullptr;
   if (c->opt.smoother == 1) {
     cg = &c->cg32;
@@ -514,55 +517,48 @@ int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double
                           double* bytes, const char** kernel_name) {
   if (!c || reps < 1) return fail(PEPSMG_ERR_ARG, "invalid argument");
   if (precision == 1 && c->h32.lv.empty()) return fail(PEPSMG_ERR_ARG, "FP32 hierarchy not built");
+  static thread_local char name[96];
   float ms = 0;
-  double n = 0, es = precision == 1 ? 4.0 : 8.0, per = 0;
-  const char* name = "";
-  bool fused = c->opt.fused_smoother != 0;
-  int ty_opt = c->opt.tile_y;
+  double n = 0;
+  const bool fused = c->opt.fused_smoother != 0;
   // fused: one red+black iteration reads x and b and writes x once (3 n
   // elements); colour sweep: reads x and b, writes half of x (2 n)
-  auto body = [&](auto& op, auto& d) {
-    for (int q = 0; q < reps; q++) {
-      if (fused)
-        op.fused_iter(d, (decltype(d.x[0] + 0))omega, 0);
-      else
-        op.smooth_colour(d, d.b, d.x, (decltype(d.x[0] + 0))omega, q & 1, false);
-    }
+  auto run = [&](auto& op, auto& d, const char* tname) -> int {
+    op.tile_y = c->opt.tile_y;
+    op.tile_x = c->opt.tile_x;
+    op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
+    using T = typename std::remove_reference<decltype(d.x[0])>::type;
+    auto body = [&]() {
+      for (int q = 0; q < reps; q++) {
+        if (fused) op.fused_iter(d, (T)omega, 0);
+        else op.smooth_colour(d, d.b, d.x, (T)omega, q & 1, false);
+      }
+    };
+    body();
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    body();
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    n = (double)d.cells();
+    if (fused)
+      std::snprintf(name, sizeof name, "k_rb_fused<%s, %d, %d, 0, %d>", tname, op.tx_used(), op.ty_used(),
+                    op.prefetch);
+    else
+      std::snprintf(name, sizeof name, "k_rb_colour<%s, false>", tname);
+    return PEPSMG_OK;
   };
+  int rc;
   if (precision == 1) {
     Ops<float> op{c->stream};
-    int ty = ty_opt ? ty_opt : 8;
-    op.tile_y = ty;
-    op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
-    DevLevel<float>& d = c->h32.lv[0];
-    body(op, d);
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    body(op, d);
-    HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    n = (double)d.cells();
-    name = !fused ? "k_rb_colour<float, false>"
-           : c->opt.fused_smoother == 2 ? (ty == 16 ? "k_rb_fused<float, 128, 16, 0, 2>" : "k_rb_fused<float, 128, 8, 0, 2>")
-                                        : (ty == 16 ? "k_rb_fused<float, 128, 16, 0, 1>" : "k_rb_fused<float, 128, 8, 0, 1>");
+    rc = run(op, c->h32.lv[0], "float");
   } else {
     Ops<double> op{c->stream};
-    int ty = ty_opt ? ty_opt : 16;
-    op.tile_y = ty;
-    op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
-    DevLevel<double>& d = c->h64.lv[0];
-    body(op, d);
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    body(op, d);
-    HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    n = (double)d.cells();
-    name = !fused ? "k_rb_colour<double, false>"
-           : c->opt.fused_smoother == 2 ? (ty == 16 ? "k_rb_fused<double, 64, 16, 0, 2>" : "k_rb_fused<double, 64, 8, 0, 2>")
-                                        : (ty == 16 ? "k_rb_fused<double, 64, 16, 0, 1>" : "k_rb_fused<double, 64, 8, 0, 1>");
+    rc = run(op, c->h64.lv[0], "double");
   }
-  per = fused ? 3.0 : 2.0;
+  if (rc) return rc;
   HIP_TRY(hipEventSynchronize(c->ev1));
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   if (avg_ms) *avg_ms = ms / reps;
-  if (bytes) *bytes = per * n * es;
+  if (bytes) *bytes = (fused ? 3.0 : 2.0) * n * (precision == 1 ? 4.0 : 8.0);
   if (kernel_name) *kernel_name = name;
   return PEPSMG_OK;
 }

@@ -129,3 +129,46 @@ def test_batch_rejects_bad_input():
     B.destroy()
     with pytest.raises(_lib.PepsmgError):
         BatchSolver([cube(12)], [1], [1]).solve_device([0], [0], 2, 1e-10, 1.0)
+
+
+@pytest.mark.gpu
+def test_batch_per_problem_tiles_fp64_bitwise():
+    """Problems with different tile shapes run in separate launches; FP64
+    values stay those of the single-grid solves."""
+    import torch
+    from peps_mg_amd.batch import BatchSolver
+    sols, rhss = make_inputs(torch, PROBLEMS, seed=5)
+    B = BatchSolver([cube(n) for n, _, _ in PROBLEMS], [nl for _, nl, _ in PROBLEMS], [r for _, _, r in PROBLEMS],
+                    ncycle=100)
+    B.set_tiles([0, 64, 64, 0], [8, 16, 8, 0])
+    res = B.solve_device(sols, rhss, 2, 1e-10, 1.0)
+    torch.cuda.synchronize()
+    ref, rref = single_solves(torch, PROBLEMS, rhss)
+    for i in range(B.size):
+        assert np.array_equal(res[i].history, rref[i].history), i
+        assert torch.equal(sols[i], ref[i]), i
+    with pytest.raises(_lib.PepsmgError):
+        B.set_tiles([32, 0, 0, 0], None)
+    B.destroy()
+
+
+@pytest.mark.gpu
+def test_autotune_tiles_and_batch(tmp_path, monkeypatch):
+    import torch
+    from peps_mg_amd import autotune
+    from peps_mg_amd.batch import BatchSolver
+    monkeypatch.setenv("PEPSMG_TUNE_CACHE", str(tmp_path / "tune.json"))
+    t = autotune.tune_tiles(cube(32), 3, 1, reps=2)
+    assert t["tile_x"] in (64, 128) and t["tile_y"] in (8, 16) and t["smoother_ms"] > 0
+    assert autotune.tune_tiles(cube(32), 3, 1) == t            # served from the cache
+    probs = [(16, 2, 1), (32, 3, 2)]
+    sols, rhss = make_inputs(torch, probs, seed=6)
+    B = BatchSolver([cube(n) for n, _, _ in probs], [nl for _, nl, _ in probs], [r for _, _, r in probs],
+                    ncycle=100, precision=1)
+    out = autotune.tune_batch(B, sols, rhss, inner_cycles=(2, 3), repeats=0)
+    assert out["inner_cycles"] in (2, 3) and "32x32x32" in out["tiles"]
+    for x in sols:
+        x.zero_()
+    res = B.solve_device(sols, rhss, 2, 1e-10, 1.0)
+    assert all(r.converged for r in res)
+    B.destroy()

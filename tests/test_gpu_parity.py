@@ -147,8 +147,9 @@ def test_vcycle_cg_smoother():
     assert np.max(np.abs(xg - xo)) <= 1e-10 * np.abs(xo).max()
 
 
+@pytest.mark.parametrize("tx", [0, 64])
 @pytest.mark.parametrize("dims,lens", [((64, 64, 64), (1, 1, 1)), ((32, 32, 32), (1, 1, 2))])
-def test_mixed_precision_vs_fp64_oracle(dims, lens):
+def test_mixed_precision_vs_fp64_oracle(dims, lens, tx):
     g = O.Grid(*dims, l=lens)
     A = g.matrix()
     b = -O.uniform_field(*dims, 7)
@@ -156,7 +157,7 @@ def test_mixed_precision_vs_fp64_oracle(dims, lens):
     mo = O.Multigrid(g, nl, 80)
     xo, ro = O.padded(*dims), O.padded(*dims)
     mo.solve_vcycle(xo, ro, A, b, 2, 1e-10, 1.0)
-    mg = Multigrid(sdm_from(g), nl, 80, precision=1, inner_cycles=4)
+    mg = Multigrid(sdm_from(g), nl, 80, precision=1, inner_cycles=4, tile_x=tx)
     xg, rg = O.padded(*dims), O.padded(*dims)
     res = mg.solve_vcycle(xg, rg, None, b, 2, 1e-10, 1.0)
     assert res.converged and res.rho < 1e-10
