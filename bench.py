@@ -402,17 +402,19 @@ def main():
     kname = C.c_char_p()
     _lib.check(_lib.lib().pepsmg_bench_smoother(mg64.h, 0, a.smoother_reps, 1.0, C.byref(avg), C.byref(nbytes),
                                                 C.byref(kname)))
+    kname64 = kname.value.decode()   # the library reuses its name buffer on the next call
     achieved = nbytes.value / (avg.value * 1e-3) / 1e9
     avg32 = C.c_double(0)
     nbytes32 = C.c_double(0)
     kname32 = C.c_char_p()
     _lib.check(_lib.lib().pepsmg_bench_smoother(mgmx.h, 1, a.smoother_reps, 1.0, C.byref(avg32),
                                                 C.byref(nbytes32), C.byref(kname32)))
+    kname32s = kname32.value.decode()
     achieved32 = nbytes32.value / (avg32.value * 1e-3) / 1e9
 
     traffic, traffic_note, tr = None, "not measured (--no-traffic)", None
     if not a.no_traffic:
-        tr, err = measure_traffic(a, kname.value.decode())
+        tr, err = measure_traffic(a, kname64)
         if tr:
             traffic = tr["FETCH_SIZE"] + tr["WRITE_SIZE"]
             traffic_note = (f"rocprofv3 FETCH_SIZE x {tr['FETCH_SIZE_scale']:.3f} + WRITE_SIZE x "
@@ -461,9 +463,9 @@ def main():
                      "traffic": round(traffic) if traffic else None, "traffic_note": traffic_note,
                      "traffic_read": round(tr["FETCH_SIZE"]) if tr else None,
                      "traffic_write": round(tr["WRITE_SIZE"]) if tr else None,
-                     "kernel": kname.value.decode(), "avg_ms": avg.value, "bytes_per_launch": nbytes.value},
+                     "kernel": kname64, "avg_ms": avg.value, "bytes_per_launch": nbytes.value},
         "roofline_fp32_smoother": {"achieved": round(achieved32, 1), "frac": round(achieved32 / HBM_PEAK_GBS, 4),
-                                   "kernel": kname32.value.decode(), "avg_ms": avg32.value,
+                                   "kernel": kname32s, "avg_ms": avg32.value,
                                    "bytes_per_launch": nbytes32.value},
         "cpu_baseline": cpu,
     }

@@ -156,7 +156,8 @@ int pepsmg_vv_dot(int32_t nx, int32_t ny, int32_t nz, const double *x, const dou
  * (precision 0: FP64, 1: FP32) on the context's stream, timing them with
  * HIP events on that stream.  *avg_ms receives the mean duration of one
  * launch, *bytes the algorithmic HBM bytes of one launch (each input read
- * once, each output written once), *kernel_name the kernel's symbol. */
+ * once, each output written once), *kernel_name the kernel's symbol (a
+ * per-thread buffer, valid until the calling thread's next call). */
 int pepsmg_bench_smoother(pepsmg_ctx *ctx, int32_t precision, int32_t reps, double omega,
                           double *avg_ms, double *bytes, const char **kernel_name);
 
