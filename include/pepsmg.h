@@ -66,6 +66,7 @@ typedef struct pepsmg_options {
   int32_t tile_y;          /* fused smoother tile height 8 or 16 cells (0: 16 in FP64, 8 in FP32) */
   int32_t verbose;         /* 1: print the reference's [MG] cycle lines */
   int32_t tile_x;          /* FP32 fused smoother tile width 64 or 128 cells (0: 128); FP64 uses 64 */
+  int32_t inner_nu;        /* precision 1: smoothing steps of the FP32 V-cycles (default 1; 0: maxiteration) */
 } pepsmg_options;
 
 /* Per-solve statistics (the reference only prints these, src/multigrid.f90:1972). */

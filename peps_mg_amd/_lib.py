@@ -32,7 +32,8 @@ class Grid(C.Structure):
 class Options(C.Structure):
     _fields_ = [("smoother", C.c_int32), ("precision", C.c_int32), ("coarse_switch", C.c_int32),
                 ("inner_cycles", C.c_int32), ("max_refine", C.c_int32), ("fused_smoother", C.c_int32),
-                ("tile_y", C.c_int32), ("verbose", C.c_int32), ("tile_x", C.c_int32)]
+                ("tile_y", C.c_int32), ("verbose", C.c_int32), ("tile_x", C.c_int32),
+                ("inner_nu", C.c_int32)]
 
 
 class Stats(C.Structure):

@@ -126,7 +126,7 @@ kb_resid(const ResidItem* __restrict__ items, int n, const int* __restrict__ act
   const int q = batch_find(items, n, blockIdx.x);
   const ResidItem& it = items[q];
   if (!active[it.item]) return;
-  resid_march_block<double, 8, KIND>(blockIdx.x - it.block0, it.x, it.e, it.b, (double*)nullptr, it.r32, it.xnew,
+  resid_march_block<double, RESID_TY, KIND>(blockIdx.x - it.block0, it.x, it.e, it.b, (double*)nullptr, it.r32, it.xnew,
                                      it.L, it.G, it.partial);
 }
 
@@ -289,7 +289,7 @@ struct pepsmg_batch {
 namespace pepsmg {
 
 // fine-level residual-march grid (same decomposition as the single-grid solver)
-static MarchGrid resid_grid(const DevLevel<double>& f) { return Ops<double>::march_grid(f, 8, false); }
+static MarchGrid resid_grid(const DevLevel<double>& f) { return Ops<double>::march_grid(f, RESID_TY, false); }
 
 // Record nu fused iterations on level l of every item that has it (per-level
 // levels only), swapping each item's x / x2 as the launches do.  Items are
@@ -592,6 +592,7 @@ int pepsmg_batch_set_options(pepsmg_batch* B, const pepsmg_options* opt) {
     return fail(PEPSMG_ERR_ARG, "batched solves use the fused smoother (fused_smoother = 1 or 2)");
   if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16) return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8 or 16");
   if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
+  if (opt->inner_nu < 0) return fail(PEPSMG_ERR_ARG, "inner_nu must be >= 0");
   if (opt->inner_cycles < 1 || opt->max_refine < 1) return fail(PEPSMG_ERR_ARG, "inner_cycles/max_refine must be >= 1");
   if (opt->coarse_switch < 1) return fail(PEPSMG_ERR_ARG, "batched solves need the persistent coarse kernel (coarse_switch > 0)");
   B->opt = *opt;
@@ -679,7 +680,8 @@ static int batch_build(pepsmg_batch* B, int nu, double tol, double omega) {
     for (auto& it : B->items) H.push_back(&it.h32);
     for (int p = 0; p < 2; p++) {
       Sched& S = B->sched[p];
-      for (int q = 0; q < B->opt.inner_cycles; q++) rec_vcycle<float>(B, S, H, nu, (float)omega, tol, q == 0);
+      const int nui = B->opt.inner_nu > 0 ? B->opt.inner_nu : nu;
+      for (int q = 0; q < B->opt.inner_cycles; q++) rec_vcycle<float>(B, S, H, nui, (float)omega, tol, q == 0);
       rec_resid<2>(B, S, true);
       rec_check(B, S, tol, maxsteps, 0);
     }

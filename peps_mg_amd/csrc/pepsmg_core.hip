@@ -313,6 +313,7 @@ void pepsmg_default_options(pepsmg_options* o) {
   o->tile_y = 0;
   o->verbose = 0;
   o->tile_x = 0;
+  o->inner_nu = 1;
 }
 
 int pepsmg_multigrid_create(const pepsmg_grid* grid, int32_t nlevel, int32_t ncycle, int32_t aggr_method,
@@ -360,6 +361,7 @@ int pepsmg_set_options(pepsmg_ctx* ctx, const pepsmg_options* opt) {
   if (opt->fused_smoother < 0 || opt->fused_smoother > 2) return fail(PEPSMG_ERR_ARG, "fused_smoother must be 0, 1 or 2");
   if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16) return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8 or 16");
   if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
+  if (opt->inner_nu < 0) return fail(PEPSMG_ERR_ARG, "inner_nu must be >= 0");
   ctx->opt = *opt;
   if (opt->precision == 1 && ctx->h32.lv.empty()) {
     int rc = ctx->h32.build(ctx->host);

@@ -14,6 +14,9 @@
 
 namespace pepsmg {
 
+// tile height of the fine-level residual kernels (norm, refinement update)
+constexpr int RESID_TY = 16;
+
 // ---------------------------------------------------------------------------
 // Launch helpers
 static inline dim3 grid3(int nx, int ny, int nz, int bx, int by) {
@@ -148,17 +151,17 @@ struct Ops {
   // |b - A x|^2 into out[0] (partial sums per block, then one block); r
   // stored when non-null.
   void resid_norm_march(DevLevel<T>& d, const T* x, const T* b, T* r, double* partial, double* out) {
-    MarchGrid G = march_grid(d, 8, false);
+    MarchGrid G = march_grid(d, RESID_TY, false);
     int nb = G.gx * G.gy * G.nch;
     if (nb > PARTIAL_CAP) {
       resid_norm(d, x, b, r, partial, out);
       return;
     }
     if (r)
-      hipLaunchKernelGGL((k_resid_march<T, 8, 1>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, s, x,
+      hipLaunchKernelGGL((k_resid_march<T, RESID_TY, 1>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, s, x,
                          (const float*)nullptr, b, r, (float*)nullptr, (T*)nullptr, d.L(), G, partial);
     else
-      hipLaunchKernelGGL((k_resid_march<T, 8, 0>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, s, x,
+      hipLaunchKernelGGL((k_resid_march<T, RESID_TY, 0>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, s, x,
                          (const float*)nullptr, b, (T*)nullptr, (float*)nullptr, (T*)nullptr, d.L(), G, partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
   }

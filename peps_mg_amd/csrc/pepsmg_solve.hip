@@ -301,11 +301,11 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   }
   DevLevel<double>& f = h.lv[0];
   DevLevel<float>& f32 = g.lv[0];
-  MarchGrid G = Ops<double>::march_grid(f, 8, false);
+  MarchGrid G = Ops<double>::march_grid(f, RESID_TY, false);
   int nb = G.gx * G.gy * G.nch;
   // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x, x2
   auto refine_update = [&]() {
-    hipLaunchKernelGGL((k_resid_march<double, 8, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
+    hipLaunchKernelGGL((k_resid_march<double, RESID_TY, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
                        f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
     std::swap(f.x, f.x2);
@@ -321,6 +321,9 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   int ncyc = 0, nref = 0;
   double rho = 1.0;
   float om = (float)omega;
+  // smoothing steps of the FP32 V-cycles (the FP64 residual and update do
+  // not depend on them): inner_nu, or the caller's maxiteration
+  const int nui = c->opt.inner_nu > 0 ? c->opt.inner_nu : nu;
   // Per-V-cycle reduction factor of the previous step.  A step that is
   // predicted (with a factor-2 margin) to reach the tolerance in fewer than
   // inner_cycles cycles runs only those: typically the last step, which needs
@@ -341,10 +344,10 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     double rho_prev = rho;
     for (int q = 0; q < nin; q++) {
       int* slot = (ncyc < c->citers_cap) ? c->d_citers + ncyc : nullptr;
-      smooth_level(c, op, g, cg, 0, nu, om, q == 0, false);
+      smooth_level(c, op, g, cg, 0, nui, om, q == 0, false);
       op.residual_restrict(f32, g.lv[1], f32.x, f32.b);
-      vcycle_coarse_part<float>(c, op, g, cg, nu, om, tol, slot);
-      smooth_level(c, op, g, cg, 0, nu, om, false, true);
+      vcycle_coarse_part<float>(c, op, g, cg, nui, om, tol, slot);
+      smooth_level(c, op, g, cg, 0, nui, om, false, true);
       ncyc++;
     }
     refine_update();

@@ -16,7 +16,7 @@ module pepsmg_c
     type, bind(C) :: pepsmg_options_t
         integer(c_int32_t) :: smoother, precision, coarse_switch, inner_cycles
         integer(c_int32_t) :: max_refine, fused_smoother, tile_y, verbose
-        integer(c_int32_t) :: tile_x
+        integer(c_int32_t) :: tile_x, inner_nu
     end type pepsmg_options_t
 
     !> struct pepsmg_stats
