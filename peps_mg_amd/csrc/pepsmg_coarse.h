@@ -100,7 +100,7 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
     for (int q = 0; q < NP; q++) {
       // an odd column <= nx keeps the 2-element load inside the padded row
       int gi = act[q] ? ia : 1, gj = act[q] ? j0 + ly[q] : 1;
-      dst[q] = *reinterpret_cast<const V2*>(xf + lidx(F, gi, gj, fc));
+      dst[q] = ld_stream_v2(xf + lidx(F, gi, gj, fc));
     }
   };
 
@@ -151,7 +151,7 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
           xv[q].y = xv[q].y + num / ((sx + nx) * (sy + ny) * (wsz + wnz));
         }
         long q0 = lidx(F, ia, j0 + ly[q], f);
-        if (two[q]) *reinterpret_cast<V2*>(xf + q0) = xv[q];
+        if (two[q]) st_stream_pair(xf + q0, xv[q].x, xv[q].y);
         else xf[q0] = xv[q].x;
       }
     }

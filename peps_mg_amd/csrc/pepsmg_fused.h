@@ -47,6 +47,17 @@ struct Vec2<double> { using type = double2; };
 template <>
 struct Vec2<float> { using type = float2; };
 
+// two adjacent elements (16-byte / 8-byte aligned) loaded non-temporally
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type ld_stream_v2(const T* p) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 r = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  typename Vec2<T>::type o;
+  o.x = r.x;
+  o.y = r.y;
+  return o;
+}
+
 struct FusedGrid {
   int gx, gy, nch, kchunk;
 };
@@ -264,10 +275,7 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
       const bool bf = Z ? !b_odd[q] : b_odd[q];   // black is the first cell iff gj + k odd
       // the pair is stored whole: a second cell beyond nx lies in the row's
       // padding (odd column <= nx, pitch a multiple of 16) and gets 0
-      V2 w;
-      w.x = bf ? vb : vr;
-      w.y = bf ? vr : vb;
-      *reinterpret_cast<V2*>(out + b_so[q]) = w;
+      st_stream_pair(out + b_so[q], bf ? vb : vr, bf ? vr : vb);
     }
   };
 

@@ -63,6 +63,20 @@ __device__ __forceinline__ T ldc(const T* p, int i) {
   return ((const __attribute__((address_space(4))) T*)p)[i];
 }
 
+// Streaming accesses for data a kernel touches once (right-hand sides,
+// outputs): marked non-temporal so they do not evict the x lines that
+// neighbouring tiles re-read from L2.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ void st_stream_pair(T* p, T a, T b) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 o = {a, b};
+  __builtin_nontemporal_store(o, reinterpret_cast<v2*>(p));
+}
+
 template <typename T>
 __device__ __forceinline__ long lidx(const Lvl<T>& L, int i, int j, int k) {
   return (long)(k - 1) * L.pxy + (long)(j - 1) * L.px + (i - 1);

@@ -144,7 +144,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
 #pragma unroll
     for (int q = 0; q < NP; q++) {
       int gi = act[q] ? i0 + 2 * lp[q] : 1, gj = act[q] ? j0 + ly[q] : 1;
-      bpre[q] = *reinterpret_cast<const V2*>(b + lidx(L, gi, gj, mc));
+      bpre[q] = ld_stream_v2(b + lidx(L, gi, gj, mc));
     }
   };
 
@@ -186,7 +186,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
           V2 w;
           w.x = ra;
           w.y = rb;
-          *reinterpret_cast<V2*>(rout + q0) = w;
+          st_stream_pair(rout + q0, w.x, w.y);
         } else {
           rout[q0] = ra;
         }
@@ -196,11 +196,11 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
           V2 w;
           w.x = s0[ly[q] + 1][2 * lp[q] + 2];
           w.y = s0[ly[q] + 1][2 * lp[q] + 3];
-          *reinterpret_cast<V2*>(xnew + q0) = w;
+          st_stream_pair(xnew + q0, w.x, w.y);
           float2 f;
           f.x = (float)ra;
           f.y = (float)rb;
-          *reinterpret_cast<float2*>(r32 + q0) = f;
+          st_stream_pair(r32 + q0, f.x, f.y);
         } else {
           xnew[q0] = s0[ly[q] + 1][2 * lp[q] + 2];
           r32[q0] = (float)ra;
@@ -283,7 +283,7 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
 #pragma unroll
     for (int q = 0; q < NP; q++) {
       int gi = act[q] ? i0 + 2 * lp[q] : 1, gj = act[q] ? j0 + ly[q] : 1;
-      bpre[q] = *reinterpret_cast<const V2*>(b + lidx(L, gi, gj, mc));
+      bpre[q] = ld_stream_v2(b + lidx(L, gi, gj, mc));
     }
   };
 
@@ -333,7 +333,7 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
         T s = v111 * sres[1][lyp][lxp] + v211 * sres[1][lyp][lxz] + v121 * sres[1][lyz][lxp] +
               v221 * sres[1][lyz][lxz] + v112 * sres[0][lyp][lxp] + v212 * sres[0][lyp][lxz] +
               v122 * sres[0][lyz][lxp] + v222 * sres[0][lyz][lxz];
-        bc[lidx(Cc, ic, jc, kc)] = s / vol_c;
+        __builtin_nontemporal_store(s / vol_c, bc + lidx(Cc, ic, jc, kc));
       }
     }
   };
