@@ -412,6 +412,11 @@ def main():
     kname32s = kname32.value.decode()
     achieved32 = nbytes32.value / (avg32.value * 1e-3) / 1e9
 
+    # the bandwidth a kernel with the smoother's streams (x, b in; x out) and no
+    # stencil reaches on this GPU: the practical ceiling for the smoother
+    s_ms, s_bytes = C.c_double(0), C.c_double(0)
+    _lib.check(_lib.lib().pepsmg_bench_stream(n ** 3, 10, C.byref(s_ms), C.byref(s_bytes)))
+    stream_gbs = s_bytes.value / (s_ms.value * 1e-3) / 1e9
     traffic, traffic_note, tr = None, "not measured (--no-traffic)", None
     if not a.no_traffic:
         tr, err = measure_traffic(a, kname64)
@@ -463,7 +468,9 @@ def main():
                      "traffic": round(traffic) if traffic else None, "traffic_note": traffic_note,
                      "traffic_read": round(tr["FETCH_SIZE"]) if tr else None,
                      "traffic_write": round(tr["WRITE_SIZE"]) if tr else None,
-                     "kernel": kname64, "avg_ms": avg.value, "bytes_per_launch": nbytes.value},
+                     "kernel": kname64, "avg_ms": avg.value, "bytes_per_launch": nbytes.value,
+                     "stream_ceiling_gbs": round(stream_gbs, 1),
+                     "frac_of_stream_ceiling": round(achieved / stream_gbs, 4)},
         "roofline_fp32_smoother": {"achieved": round(achieved32, 1), "frac": round(achieved32 / HBM_PEAK_GBS, 4),
                                    "kernel": kname32s, "avg_ms": avg32.value,
                                    "bytes_per_launch": nbytes32.value},

@@ -162,6 +162,14 @@ int pepsmg_vv_dot(int32_t nx, int32_t ny, int32_t nz, const double *x, const dou
 int pepsmg_bench_smoother(pepsmg_ctx *ctx, int32_t precision, int32_t reps, double omega,
                           double *avg_ms, double *bytes, const char **kernel_name);
 
+/* Measurement hook (no counterpart in the reference): the bandwidth a
+ * kernel with the smoother's access pattern but no stencil reaches, i.e.
+ * y = x/2 + b over n doubles (two streams read, one written, 16 bytes per
+ * lane), launched `reps` times on the default stream after `reps` warm-up
+ * launches.  Allocates and frees its three arrays.  *avg_ms: mean launch
+ * time; *bytes: 3 n 8. */
+int pepsmg_bench_stream(int64_t n, int32_t reps, double *avg_ms, double *bytes);
+
 /* Synthetic workload input (no counterpart in the reference, which has no
  * random right-hand side): writes into a padded device array the field
  *   f(i,j,k) = scale_rand * (2u-1) + scale_sin * sin(pi x) sin(pi y) sin(pi z)

@@ -400,6 +400,57 @@ int pepsmg_fill_rhs_host(int32_t nx, int32_t ny, int32_t nz, double* out, uint64
   return PEPSMG_OK;
 }
 
+// y = x / 2 + b over n doubles, 16 bytes per lane, output stored
+// non-temporally: the access pattern of one smoother launch (two streams in,
+// one out) without the stencil, i.e. the bandwidth such a kernel can reach.
+__global__ void __launch_bounds__(256) k_stream_triad(double* __restrict__ y, const double* __restrict__ x,
+                                                      const double* __restrict__ b, long n2) {
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  long k = (long)blockIdx.x * 256 + threadIdx.x;
+  if (k < n2) {
+    v2 a = reinterpret_cast<const v2*>(x)[k], c = reinterpret_cast<const v2*>(b)[k];
+    v2 r = a * 0.5 + c;
+    __builtin_nontemporal_store(r, reinterpret_cast<v2*>(y) + k);
+  }
+}
+
+int pepsmg_bench_stream(int64_t n, int32_t reps, double* avg_ms, double* bytes) {
+  if (n < 2 || reps < 1) return fail(PEPSMG_ERR_ARG, "invalid argument");
+  n &= ~1LL;
+  double *x = nullptr, *b = nullptr, *y = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PEPSMG_OK;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == PEPSMG_OK) rc = hip_fail(e, what);
+  };
+  chk(hipMalloc(&x, n * sizeof(double)), "hipMalloc");
+  chk(hipMalloc(&b, n * sizeof(double)), "hipMalloc");
+  chk(hipMalloc(&y, n * sizeof(double)), "hipMalloc");
+  chk(hipEventCreate(&e0), "hipEventCreate");
+  chk(hipEventCreate(&e1), "hipEventCreate");
+  if (rc == PEPSMG_OK) {
+    chk(hipMemset(x, 0, n * sizeof(double)), "hipMemset");
+    chk(hipMemset(b, 0, n * sizeof(double)), "hipMemset");
+    long n2 = n / 2;
+    dim3 g((unsigned)((n2 + 255) / 256));
+    for (int q = 0; q < reps; q++) hipLaunchKernelGGL(k_stream_triad, g, dim3(256), 0, 0, y, x, b, n2);
+    chk(hipEventRecord(e0, 0), "hipEventRecord");
+    for (int q = 0; q < reps; q++) hipLaunchKernelGGL(k_stream_triad, g, dim3(256), 0, 0, y, x, b, n2);
+    chk(hipEventRecord(e1, 0), "hipEventRecord");
+    chk(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    chk(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    if (avg_ms) *avg_ms = ms / reps;
+    if (bytes) *bytes = 3.0 * n * sizeof(double);
+  }
+  hipFree(x);
+  hipFree(b);
+  hipFree(y);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  return rc;
+}
+
 int pepsmg_fill_rhs_device(int32_t nx, int32_t ny, int32_t nz, double* d_out, uint64_t seed, double scale_rand,
                            double scale_sin, const double* xg, const double* yg, const double* zg, void* stream) {
   if (nx < 1 || ny < 1 || nz < 1 || !d_out) return fail(PEPSMG_ERR_ARG, "invalid argument");
