@@ -4,13 +4,19 @@
 !>
 !>   &gpu smoother = 0|1 (RBGS | CG), precision = 0|1 (FP64 | FP32 V-cycles in
 !>        FP64 refinement), coarse_switch, inner_cycles, max_refine, verbose,
-!>        rhs_mode = 0|1, seed, output_file /
+!>        rhs_mode = 0|1, seed, output_file, write_solution /
 !>
 !> rhs_mode 0 is the reference's "Problem 1" cosine right-hand side with its
 !> RMS check (src/main_peps.f90:78-90, :172-182); rhs_mode 1 is the synthetic
 !> workload b = -(U[-1,1] + 3 pi^2 sin(pi x) sin(pi y) sin(pi z)) with U the
 !> SplitMix64 field of the given seed.  output_file, if set, receives the
-!> padded solution array as raw float64 (stream access).
+!> padded solution array as raw float64 (stream access).  write_solution
+!> (default .false.) writes the reference's solution files
+!> (file_write, src/main_peps.f90:204-446, on one process): solution_all.dat
+!> (x, y, z, u per interior cell, format (3D20.10,1E30.20)), solution_all.bin
+!> (one unformatted record of u(1:nx,1:ny,1:nz)) and
+!> solution_all_MPI_IO.bin (the same values as raw float64), then checks
+!> that the two binary files agree (binary_file_check, :448-484).
 program main_peps
 
     use, intrinsic :: iso_c_binding
@@ -35,6 +41,7 @@ program main_peps
     integer(kind=4) :: smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, rhs_mode
     integer(kind=8) :: seed
     character(len=256) :: output_file
+    logical :: write_solution
 
     type(domain), target     :: g_domain
     type(subdomain), target  :: s_domain
@@ -52,10 +59,11 @@ program main_peps
     namelist /control/ maxiteration, tolerance, number_of_vcycles, number_of_levels, aggregation_method, &
                        CGA_level, CGPSA_level, omega_sor
     namelist /coefficients/ alpha_x, alpha_y, alpha_z
-    namelist /gpu/ smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, rhs_mode, seed, output_file
+    namelist /gpu/ smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, rhs_mode, seed, output_file, &
+                   write_solution
 
     smoother = 0; precision = 0; coarse_switch = 4096; inner_cycles = 3; max_refine = 30; verbose = 0
-    rhs_mode = 0; seed = 20210501_8; output_file = ''
+    rhs_mode = 0; seed = 20210501_8; output_file = ''; write_solution = .false.
     alpha_x = 1.0d0; alpha_y = 1.0d0; alpha_z = 1.0d0
 
     open(unit=1, file="PARA_INPUT.inp", status='old')
@@ -134,6 +142,12 @@ program main_peps
         print '(a,a)', '[Poisson] Solution written to ', trim(output_file)
     endif
 
+    if (write_solution) then
+        call file_write
+        call binary_file_check
+        print '(a)', '[Poisson] Solution printed.'
+    endif
+
     deallocate(ref_sub)
     call matrix_heptadiagonal_destroy(a_poisson)
     call mpi_topology_destroy
@@ -141,5 +155,51 @@ program main_peps
     call geometry_subdomain_destroy(s_domain)
     call geometry_domain_destroy(g_domain)
     print '(a)', '[Poisson] Memory deallocated.'
+
+contains
+
+    !> The reference's three solution files for one process (the gathers of
+    !> src/main_peps.f90:204-446 reduce to the subdomain's interior).
+    subroutine file_write
+        integer(kind=4) :: i, j, k, uf
+        open(newunit=uf, form="formatted", file="solution_all.dat", status='replace')
+        do k = 1, g_domain%nz
+            do j = 1, g_domain%ny
+                do i = 1, g_domain%nx
+                    write(uf, '(3D20.10,1E30.20)') g_domain%xg(i), g_domain%yg(j), g_domain%zg(k), s_domain%x(i, j, k)
+                enddo
+            enddo
+        enddo
+        close(uf)
+        open(newunit=uf, form="unformatted", file="solution_all.bin", status='replace')
+        write(uf) s_domain%x(1:s_domain%nx, 1:s_domain%ny, 1:s_domain%nz)
+        close(uf)
+        open(newunit=uf, form="unformatted", access='stream', file="solution_all_MPI_IO.bin", status='replace')
+        write(uf) s_domain%x(1:s_domain%nx, 1:s_domain%ny, 1:s_domain%nz)
+        close(uf)
+    end subroutine file_write
+
+    !> Reads both binary files back and prints every cell where they differ
+    !> (src/main_peps.f90:448-484).
+    subroutine binary_file_check
+        real(kind=8), allocatable :: temp1(:,:,:), temp2(:,:,:)
+        integer(kind=4) :: i, j, k, uf
+        allocate(temp1(1:g_domain%nx, 1:g_domain%ny, 1:g_domain%nz))
+        allocate(temp2(1:g_domain%nx, 1:g_domain%ny, 1:g_domain%nz))
+        open(newunit=uf, form='unformatted', file="solution_all.bin", status='old')
+        read(uf) temp1
+        close(uf)
+        open(newunit=uf, form='unformatted', access='stream', file="solution_all_MPI_IO.bin", status='old')
+        read(uf) temp2
+        close(uf)
+        do k = 1, g_domain%nz
+            do j = 1, g_domain%ny
+                do i = 1, g_domain%nx
+                    if (temp1(i, j, k) /= temp2(i, j, k)) print *, i, j, k, temp1(i, j, k), temp2(i, j, k)
+                enddo
+            enddo
+        enddo
+        deallocate(temp1, temp2)
+    end subroutine binary_file_check
 
 end program main_peps

@@ -133,3 +133,29 @@ def test_driver_level_error_stops(tmp_path):
     p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert p.returncode != 0
     assert "multigrid_create failed" in p.stdout
+
+
+@pytest.mark.gpu
+def test_driver_writes_reference_solution_files(tmp_path):
+    """write_solution: the reference's solution_all.dat / .bin / _MPI_IO.bin
+    (src/main_peps.f90:204-446) hold the interior of the solved field."""
+    exe = ensure_built()
+    n = 12
+    write_input(tmp_path, n=n, nl=2, gpu=dict(rhs_mode=1, seed=3, output_file="'x.bin'", write_solution=".true."))
+    p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "Solution printed" in p.stdout
+    x = np.fromfile(tmp_path / "x.bin", dtype=np.float64).reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]
+    raw = np.fromfile(tmp_path / "solution_all_MPI_IO.bin", dtype=np.float64).reshape(n, n, n)
+    assert np.array_equal(raw, x)
+    rec = (tmp_path / "solution_all.bin").read_bytes()
+    size = np.frombuffer(rec[:4], dtype=np.int32)[0]
+    assert size == n ** 3 * 8 and np.frombuffer(rec[-4:], dtype=np.int32)[0] == size
+    assert np.array_equal(np.frombuffer(rec[4:-4], dtype=np.float64).reshape(n, n, n), x)
+    lines = (tmp_path / "solution_all.dat").read_text().splitlines()
+    assert len(lines) == n ** 3
+    first = [float(v.replace("D", "E")) for v in lines[0].split()]
+    assert abs(first[3] - x[0, 0, 0]) <= 1e-18 + 1e-19 * abs(x[0, 0, 0])
+    last = [float(v.replace("D", "E")) for v in lines[-1].split()]
+    assert abs(last[3] - x[-1, -1, -1]) <= 1e-18 + 1e-19 * abs(x[-1, -1, -1])
+    assert len(lines[0]) == 3 * 20 + 30
