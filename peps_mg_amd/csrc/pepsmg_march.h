@@ -15,6 +15,10 @@
 //   k_resid_march<T,TY,1>: same, and stores r
 //   k_resid_march<double,TY,2>: iterative-refinement step: x_in = x + e
 //       (e the FP32 correction), stores x_in, r as float, partial r^2
+//   k_resid_march<T,TY,3>: CG step Ap = A p (stored in rout) and partial sums
+//                          of (Ap, p) (src/cg_poisson_matrix.f90:206-207)
+//   k_resid_march<T,TY,4>: CG start r = b - A x stored in rout and in xnew (p),
+//                          partial sums of (r, r) (:196-204)
 //   k_restrict_march<T>:   residual + restriction into the next level's b
 //       (src/multigrid.f90:1932-1933, :1939-1940 with :1460-1494), for
 //       levels that halve every axis
@@ -86,14 +90,20 @@ struct XStager {
 
 // r at tile cell (lx, ly) of plane slots (s0 = m, sm = m-1, sp = m+1);
 // LDS column of cell lx is lx+2, row ly+1.
+// (A x) at tile cell (lx, ly), in mv_mul's order (c0 term first)
+template <typename T, int RX>
+__device__ __forceinline__ T march_apply(T (*s0)[RX], T (*sm)[RX], T (*sp)[RX], int lx, int ly, T c1, T c2, T c3,
+                                         T c4, T c5, T c6) {
+  T c0 = diag(c1, c2, c3, c4, c5, c6);
+  int cx = lx + 2, cy = ly + 1;
+  return c0 * s0[cy][cx] + c1 * s0[cy][cx - 1] + c2 * s0[cy][cx + 1] + c3 * s0[cy - 1][cx] + c4 * s0[cy + 1][cx] +
+         c5 * sm[cy][cx] + c6 * sp[cy][cx];
+}
+
 template <typename T, int RX>
 __device__ __forceinline__ T march_resid(T (*s0)[RX], T (*sm)[RX], T (*sp)[RX], int lx, int ly, T c1, T c2, T c3,
                                          T c4, T c5, T c6, T bv) {
-  T c0 = diag(c1, c2, c3, c4, c5, c6);
-  int cx = lx + 2, cy = ly + 1;
-  T y = c0 * s0[cy][cx] + c1 * s0[cy][cx - 1] + c2 * s0[cy][cx + 1] + c3 * s0[cy - 1][cx] + c4 * s0[cy + 1][cx] +
-        c5 * sm[cy][cx] + c6 * sp[cy][cx];
-  return bv - y;
+  return bv - march_apply<T, RX>(s0, sm, sp, lx, ly, c1, c2, c3, c4, c5, c6);
 }
 
 template <typename T, int TY, int KIND>
@@ -140,6 +150,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
   // b of this thread's pairs, from clamped addresses (values of cells
   // outside the domain are never used)
   auto issue_b = [&](int m) {
+    if (KIND == 3) return;                   // A p needs no right-hand side
     const int mc = min(max(m, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
@@ -175,21 +186,27 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
       if (!act[q]) continue;
       int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
       bool two = gi + 1 <= L.nx;
-      T ra = march_resid<T, RX>(s0, sm, sp, 2 * lp[q], ly[q], c1a[q], c2a[q], c3[q], c4[q], c5, c6, bv[q].x);
-      T rb = two ? march_resid<T, RX>(s0, sm, sp, 2 * lp[q] + 1, ly[q], c1b[q], c2b[q], c3[q], c4[q], c5, c6, bv[q].y)
+      T ra, rb;
+      if (KIND == 3) {
+        ra = march_apply<T, RX>(s0, sm, sp, 2 * lp[q], ly[q], c1a[q], c2a[q], c3[q], c4[q], c5, c6);
+        rb = two ? march_apply<T, RX>(s0, sm, sp, 2 * lp[q] + 1, ly[q], c1b[q], c2b[q], c3[q], c4[q], c5, c6) : T(0);
+        acc += (double)ra * (double)s0[ly[q] + 1][2 * lp[q] + 2];
+        if (two) acc += (double)rb * (double)s0[ly[q] + 1][2 * lp[q] + 3];
+      } else {
+        ra = march_resid<T, RX>(s0, sm, sp, 2 * lp[q], ly[q], c1a[q], c2a[q], c3[q], c4[q], c5, c6, bv[q].x);
+        rb = two ? march_resid<T, RX>(s0, sm, sp, 2 * lp[q] + 1, ly[q], c1b[q], c2b[q], c3[q], c4[q], c5, c6, bv[q].y)
                  : T(0);
-      acc += (double)ra * (double)ra;
-      acc += (double)rb * (double)rb;
+        acc += (double)ra * (double)ra;
+        acc += (double)rb * (double)rb;
+      }
       long q0 = lidx(L, gi, gj, m);
-      if (KIND == 1) {
-        if (two) {
-          V2 w;
-          w.x = ra;
-          w.y = rb;
-          st_stream_pair(rout + q0, w.x, w.y);
-        } else {
-          rout[q0] = ra;
-        }
+      if (KIND == 1 || KIND == 3 || KIND == 4) {
+        if (two) st_stream_pair(rout + q0, ra, rb);
+        else rout[q0] = ra;
+      }
+      if (KIND == 4) {
+        if (two) st_stream_pair(xnew + q0, ra, rb);
+        else xnew[q0] = ra;
       }
       if (KIND == 2) {
         if (two) {

@@ -17,74 +17,128 @@
 namespace pepsmg {
 
 // ---------------------------------------------------------------------------
-// CG iterator (src/cg_poisson_matrix.f90:157-241) on device arrays.  The
-// scalars live on the device: sc[0] = (r,r) at the start of the iteration
-// (temp1), sc[1] = (Ap,p), sc[2] = (r,r) after the update (temp2).
-template <typename T>
-__global__ void k_cg_init(T* __restrict__ r, T* __restrict__ p, const T* __restrict__ x, const T* __restrict__ b,
-                          Lvl<T> L, bool zero_x) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > L.nx || j > L.ny) return;
-  long q = lidx(L, i, j, k);
-  T v;
-  if (zero_x) {
-    // A*0 evaluated in the reference order is +0, and b - 0 = b
-    v = b[q] - T(0);
-  } else {
-    v = resid_at(x, b, L, i, j, k);
+// CG iterator (src/cg_poisson_matrix.f90:157-241) on device arrays, four
+// passes per iteration: Ap = A p with the partial sums of (Ap, p) (one
+// z-march pass), x += alpha p and r -= alpha Ap with the partial sums of
+// (r, r) (one elementwise pass), p = r + beta p (one pass, skipped after the
+// last iteration: p is local to the reference routine), and the final
+// reductions.  Scalars stay on the device: sc[it & 1] = (r, r) at the start
+// of iteration it (temp1), sc[2] = (Ap, p) (temp2); alpha and beta are
+// formed in FP64 by every thread as the reference does (:209, :223).
+constexpr int CG_WAVES = 4;
+
+// rows (j, k) of a level dealt to waves; each lane handles pairs (i odd, i+1)
+template <typename T, typename F>
+__device__ __forceinline__ double cg_rows(const Lvl<T>& L, F&& f) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nrow = L.ny * L.nz, npair = (L.nx + 1) / 2;
+  double acc = 0.0;
+  for (int row = blockIdx.x * CG_WAVES + w; row < nrow; row += gridDim.x * CG_WAVES) {
+    long base = (long)(row / L.ny) * L.pxy + (long)(row % L.ny) * L.px;
+    for (int pp = lane; pp < npair; pp += 64) acc += f(base + 2 * pp, 2 * pp + 2 <= L.nx);
   }
-  r[q] = v;
-  p[q] = v;
+  return acc;
 }
 
+// r = b - A 0 = b, p = r, partial (r, r) (x taken as 0, :196-204)
 template <typename T>
-__global__ void k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap,
-                            const double* __restrict__ sc, Lvl<T> L, bool zero_x) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > L.nx || j > L.ny) return;
-  long q = lidx(L, i, j, k);
-  T alpha = (T)(sc[0] / sc[1]);                      // :209
-  T xo = zero_x ? T(0) : x[q];
-  x[q] = xo + alpha * p[q];                          // :215
-  r[q] = r[q] - alpha * ap[q];                       // :216
+__global__ void __launch_bounds__(64 * CG_WAVES)
+k_cg_start0(T* __restrict__ r, T* __restrict__ p, const T* __restrict__ b, Lvl<T> L, double* __restrict__ partial) {
+  __shared__ double sh[CG_WAVES];
+  double acc = cg_rows(L, [&](long q, bool two) {
+    T r0 = b[q] - T(0), r1 = two ? b[q + 1] - T(0) : T(0);
+    r[q] = r0;
+    p[q] = r0;
+    if (two) {
+      r[q + 1] = r1;
+      p[q + 1] = r1;
+    }
+    return (double)r0 * (double)r0 + (double)r1 * (double)r1;
+  });
+  double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// x = x + alpha p, r = r - alpha Ap (:209-216), partial (r, r) (:221)
 template <typename T>
-__global__ void k_cg_pupdate(T* __restrict__ p, const T* __restrict__ r, double* __restrict__ sc, Lvl<T> L) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > L.nx || j > L.ny) return;
-  long q = lidx(L, i, j, k);
-  T beta = (T)(sc[2] / sc[0]);                       // :223
-  p[q] = r[q] + beta * p[q];                         // :229
+__global__ void __launch_bounds__(64 * CG_WAVES)
+k_cg_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap,
+        const double* __restrict__ sc, int rr_slot, Lvl<T> L, bool zero_x, double* __restrict__ partial) {
+  __shared__ double sh[CG_WAVES];
+  const T alpha = (T)(sc[rr_slot] / sc[2]);
+  double acc = cg_rows(L, [&](long q, bool two) {
+    T x0 = zero_x ? T(0) : x[q];
+    T n0 = x0 + alpha * p[q];
+    T s0 = r[q] - alpha * ap[q];
+    x[q] = n0;
+    r[q] = s0;
+    double a = (double)s0 * (double)s0;
+    if (two) {
+      T x1 = zero_x ? T(0) : x[q + 1];
+      T n1 = x1 + alpha * p[q + 1];
+      T s1 = r[q + 1] - alpha * ap[q + 1];
+      x[q + 1] = n1;
+      r[q + 1] = s1;
+      a += (double)s1 * (double)s1;
+    }
+    return a;
+  });
+  double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-__global__ void k_cg_roll(double* sc) { sc[0] = sc[2]; }
+// p = r + beta p with beta = temp2 / temp1 (:223-229)
+template <typename T>
+__global__ void __launch_bounds__(64 * CG_WAVES)
+k_cg_p(T* __restrict__ p, const T* __restrict__ r, const double* __restrict__ sc, int new_slot, int old_slot, Lvl<T> L) {
+  const T beta = (T)(sc[new_slot] / sc[old_slot]);
+  cg_rows(L, [&](long q, bool two) {
+    p[q] = r[q] + beta * p[q];
+    if (two) p[q + 1] = r[q + 1] + beta * p[q + 1];
+    return 0.0;
+  });
+}
 
 template <typename T>
 static void cg_iterate(Ops<T>& op, DevLevel<T>& d, T* x, const T* b, T* r, T* p, T* ap, int nu, bool zero_x,
                        double* partial, double* sc) {
-  dim3 g = grid3(d.nx, d.ny, d.nz, 64, 4), blk(64, 4, 1);
-  hipLaunchKernelGGL((k_cg_init<T>), g, blk, 0, op.s, r, p, x, b, d.L(), zero_x);
+  const long nrow = (long)d.ny * d.nz;
+  const int eb = (int)std::max(1L, std::min<long>((nrow + CG_WAVES - 1) / CG_WAVES, RED_BLOCKS));
+  const dim3 eblk(64 * CG_WAVES);
+  MarchGrid G = Ops<T>::march_grid(d, RESID_TY, false);
+  const int mb = G.gx * G.gy * G.nch;
+  const bool march = mb <= PARTIAL_CAP;
+  // start: r = b - A x, p = r, temp1 = (r, r)  (:196-204)
+  if (zero_x) {
+    hipLaunchKernelGGL((k_cg_start0<T>), dim3(eb), eblk, 0, op.s, r, p, b, d.L(), partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, op.s, partial, eb, sc + 0);
+  } else if (march) {
+    hipLaunchKernelGGL((k_resid_march<T, RESID_TY, 4>), dim3(mb), dim3(MARCH_THREADS), 0, op.s, x, (const float*)nullptr,
+                       b, r, (float*)nullptr, p, d.L(), G, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, op.s, partial, mb, sc + 0);
+  } else {
+    op.resid_norm(d, x, b, r, partial, sc + 0);
+    hipMemcpyAsync(p, r, d.ncell * sizeof(T), hipMemcpyDeviceToDevice, op.s);
+  }
   if (nu <= 0) {
     if (zero_x) hipMemsetAsync(x, 0, d.ncell * sizeof(T), op.s);
     return;
   }
-  op.dot(d, r, r, partial, sc + 0);                                 // temp1 (:204)
   for (int it = 0; it < nu; it++) {
-    op.apply(d, ap, p);                                             // :206
-    op.dot(d, ap, p, partial, sc + 1);                              // :207
-    hipLaunchKernelGGL((k_cg_update<T>), g, blk, 0, op.s, x, r, p, ap, sc, d.L(), zero_x && it == 0);
-    op.dot(d, r, r, partial, sc + 2);                               // :221
-    hipLaunchKernelGGL((k_cg_pupdate<T>), g, blk, 0, op.s, p, r, sc, d.L());
-    // the next iteration's temp1 = (r,r) of the same r: the deterministic
-    // reduction gives exactly sc[2]
-    hipLaunchKernelGGL(k_cg_roll, dim3(1), dim3(1), 0, op.s, sc);
+    const int cur = it & 1, nxt = (it + 1) & 1;
+    if (march) {                                                    // :206-207
+      hipLaunchKernelGGL((k_resid_march<T, RESID_TY, 3>), dim3(mb), dim3(MARCH_THREADS), 0, op.s, p,
+                         (const float*)nullptr, (const T*)nullptr, ap, (float*)nullptr, (T*)nullptr, d.L(), G, partial);
+      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, op.s, partial, mb, sc + 2);
+    } else {
+      op.apply(d, ap, p);
+      op.dot(d, ap, p, partial, sc + 2);
+    }
+    hipLaunchKernelGGL((k_cg_xr<T>), dim3(eb), eblk, 0, op.s, x, r, p, ap, sc, cur, d.L(), zero_x && it == 0,
+                       partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, op.s, partial, eb, sc + nxt);   // :221
+    if (it + 1 < nu)
+      hipLaunchKernelGGL((k_cg_p<T>), dim3(eb), eblk, 0, op.s, p, r, sc, nxt, cur, d.L());
   }
 }
 
@@ -732,24 +786,29 @@ int pepsmg_cg_solver(const pepsmg_grid* grid, double* sol, const double* coeff, 
   double *p = nullptr, *ap = nullptr;
   HIP_TRY(hipMalloc(&p, o.d.ncell * sizeof(double)));
   HIP_TRY(hipMalloc(&ap, o.d.ncell * sizeof(double)));
-  dim3 g = grid3(o.d.nx, o.d.ny, o.d.nz, 64, 4), blk(64, 4, 1);
   double* sc = o.sc;
-  // src/cg_poisson_matrix.f90:81-137
-  hipLaunchKernelGGL((k_cg_init<double>), g, blk, 0, o.s, o.d.r, p, o.d.x, o.d.b, o.d.L(), false);
-  op.dot(o.d, o.d.r, o.d.r, o.partial, sc + 3);                    // rsd0tol (:94)
+  const long nrow = (long)o.d.ny * o.d.nz;
+  const int eb = (int)std::max(1L, std::min<long>((nrow + CG_WAVES - 1) / CG_WAVES, RED_BLOCKS));
+  const dim3 eblk(64 * CG_WAVES);
+  // src/cg_poisson_matrix.f90:81-137.  (r, r) at the top of an iteration
+  // (:106) is the (r, r) the previous iteration ended with (:123), the same
+  // vector: sc[it & 1] holds it, sc[2] holds (Ap, p).
+  op.resid_norm(o.d, o.d.x, o.d.b, o.d.r, o.partial, sc + 0);      // r = b - A x, rsd0tol (:94)
+  HIP_TRY(hipMemcpyAsync(p, o.d.r, o.d.ncell * sizeof(double), hipMemcpyDeviceToDevice, o.s));
   double r0 = 0, t2 = 0;
-  rc = o.scalar(3, &r0);
+  rc = o.scalar(0, &r0);
   int it;
   for (it = 0; it < maxiteration && !rc; it++) {
-    op.dot(o.d, o.d.r, o.d.r, o.partial, sc + 0);                  // :106
+    const int cur = it & 1, nxt = (it + 1) & 1;
     op.apply(o.d, ap, p);                                          // :108
-    op.dot(o.d, ap, p, o.partial, sc + 1);                         // :109
-    hipLaunchKernelGGL((k_cg_update<double>), g, blk, 0, o.s, o.d.x, o.d.r, p, ap, sc, o.d.L(), false);
-    op.dot(o.d, o.d.r, o.d.r, o.partial, sc + 2);                  // :123
-    rc = o.scalar(2, &t2);
+    op.dot(o.d, ap, p, o.partial, sc + 2);                         // :109
+    hipLaunchKernelGGL((k_cg_xr<double>), dim3(eb), eblk, 0, o.s, o.d.x, o.d.r, p, ap, sc, cur, o.d.L(), false,
+                       o.partial);                                 // :111-118
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, o.s, o.partial, eb, sc + nxt);   // :123
+    rc = o.scalar(nxt, &t2);
     if (rc) break;
     if (std::sqrt(t2 / r0) < tolerance) break;                     // :126
-    hipLaunchKernelGGL((k_cg_pupdate<double>), g, blk, 0, o.s, p, o.d.r, sc, o.d.L());
+    hipLaunchKernelGGL((k_cg_p<double>), dim3(eb), eblk, 0, o.s, p, o.d.r, sc, nxt, cur, o.d.L());
   }
   hipFree(p);
   hipFree(ap);
