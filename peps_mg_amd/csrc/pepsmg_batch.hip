@@ -94,7 +94,7 @@ struct ResidItem {
 };
 
 template <typename T, int TX, int TY, int MODE, int PF>
-__global__ void __launch_bounds__(FUSED_THREADS)
+__global__ void __launch_bounds__(fused_threads(TX, TY))
 kb_rb_fused(const FusedItem<T>* __restrict__ items, int n, const int* __restrict__ active, T omega) {
   const int q = batch_find(items, n, blockIdx.x);
   const FusedItem<T>& it = items[q];
@@ -319,31 +319,34 @@ static void rec_smooth_group(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H
   int* act = B->d_active;
   S.ops.push_back([=](hipStream_t st, const char* dev) {
     const FusedItem<T>* p = (const FusedItem<T>*)(dev + off);
-    if (zero) hipLaunchKernelGGL((kb_rb_fused<T, TX, TY, 1, 1>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
-    else hipLaunchKernelGGL((kb_rb_fused<T, TX, TY, 0, 1>), dim3(nb), dim3(FUSED_THREADS), 0, st, p, n, act, omega);
+    const dim3 nt(fused_threads(TX, TY));
+    if (zero) hipLaunchKernelGGL((kb_rb_fused<T, TX, TY, 1, 1>), dim3(nb), nt, 0, st, p, n, act, omega);
+    else hipLaunchKernelGGL((kb_rb_fused<T, TX, TY, 0, 1>), dim3(nb), nt, 0, st, p, n, act, omega);
   });
 }
 
 template <typename T>
 static void rec_smooth(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int l, int nu, T omega,
                        bool zero_first) {
-  // tile shape per item: width 64 (FP64 always) or 128 (FP32 default), height 8 or 16
-  std::vector<int> g[2][2];
+  // tile shape per item: width 64 (FP64 always) or 128 (FP32 default), height 8, 16 or 32
+  std::vector<int> g[2][3];
   for (int i = 0; i < B->nitems; i++) {
     if (l >= B->items[i].s) continue;
     int tx = B->items[i].tile_x ? B->items[i].tile_x : B->opt.tile_x;
     int ty = B->items[i].tile_y ? B->items[i].tile_y : B->opt.tile_y;
-    if (!ty) ty = sizeof(T) == 8 ? 16 : 8;
+    if (!ty) ty = 16;
     int wx = (sizeof(T) == 4 && tx != 64) ? 1 : 0;
-    g[wx][ty == 16 ? 1 : 0].push_back(i);
+    g[wx][ty == 32 ? 2 : ty == 16 ? 1 : 0].push_back(i);
   }
   for (int it = 0; it < nu; it++) {
     bool z = zero_first && it == 0;
     if (!g[0][0].empty()) rec_smooth_group<T, 64, 8>(B, S, H, g[0][0], l, z, omega);
     if (!g[0][1].empty()) rec_smooth_group<T, 64, 16>(B, S, H, g[0][1], l, z, omega);
+    if (!g[0][2].empty()) rec_smooth_group<T, 64, 32>(B, S, H, g[0][2], l, z, omega);
     if (sizeof(T) == 4) {
       if (!g[1][0].empty()) rec_smooth_group<T, Ops<T>::FTX, 8>(B, S, H, g[1][0], l, z, omega);
       if (!g[1][1].empty()) rec_smooth_group<T, Ops<T>::FTX, 16>(B, S, H, g[1][1], l, z, omega);
+      if (!g[1][2].empty()) rec_smooth_group<T, Ops<T>::FTX, 32>(B, S, H, g[1][2], l, z, omega);
     }
   }
 }
@@ -590,7 +593,8 @@ int pepsmg_batch_set_options(pepsmg_batch* B, const pepsmg_options* opt) {
   if (opt->precision < 0 || opt->precision > 1) return fail(PEPSMG_ERR_ARG, "precision must be 0 or 1");
   if (opt->fused_smoother < 1 || opt->fused_smoother > 2)
     return fail(PEPSMG_ERR_ARG, "batched solves use the fused smoother (fused_smoother = 1 or 2)");
-  if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16) return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8 or 16");
+  if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16 && opt->tile_y != 32)
+    return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8, 16 or 32");
   if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
   if (opt->inner_nu < 0) return fail(PEPSMG_ERR_ARG, "inner_nu must be >= 0");
   if (opt->inner_cycles < 1 || opt->max_refine < 1) return fail(PEPSMG_ERR_ARG, "inner_cycles/max_refine must be >= 1");
@@ -614,7 +618,8 @@ int pepsmg_batch_set_tiles(pepsmg_batch* B, const int32_t* tile_x, const int32_t
   for (int p = 0; p < B->nprob; p++) {
     int tx = tile_x ? tile_x[p] : 0, ty = tile_y ? tile_y[p] : 0;
     if (tx != 0 && tx != 64 && tx != 128) return fail(PEPSMG_ERR_ARG, "problem %d: tile_x must be 0, 64 or 128", p);
-    if (ty != 0 && ty != 8 && ty != 16) return fail(PEPSMG_ERR_ARG, "problem %d: tile_y must be 0, 8 or 16", p);
+    if (ty != 0 && ty != 8 && ty != 16 && ty != 32)
+      return fail(PEPSMG_ERR_ARG, "problem %d: tile_y must be 0, 8, 16 or 32", p);
   }
   for (auto& it : B->items) {
     it.tile_x = tile_x ? tile_x[it.prob] : 0;

@@ -359,7 +359,8 @@ int pepsmg_set_options(pepsmg_ctx* ctx, const pepsmg_options* opt) {
   if (opt->precision < 0 || opt->precision > 1) return fail(PEPSMG_ERR_ARG, "precision must be 0 or 1");
   if (opt->inner_cycles < 1 || opt->max_refine < 1) return fail(PEPSMG_ERR_ARG, "inner_cycles/max_refine must be >= 1");
   if (opt->fused_smoother < 0 || opt->fused_smoother > 2) return fail(PEPSMG_ERR_ARG, "fused_smoother must be 0, 1 or 2");
-  if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16) return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8 or 16");
+  if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16 && opt->tile_y != 32)
+    return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8, 16 or 32");
   if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
   if (opt->inner_nu < 0) return fail(PEPSMG_ERR_ARG, "inner_nu must be >= 0");
   ctx->opt = *opt;

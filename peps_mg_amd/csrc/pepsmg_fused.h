@@ -39,6 +39,10 @@
 namespace pepsmg {
 
 constexpr int FUSED_THREADS = 256;
+// threads of a TX x TY tile: two black pairs per thread (at least 256
+// threads), so taller or wider tiles keep the per-thread registers of the
+// 64 x 16 (FP64) and 128 x 8 (FP32) ones
+constexpr int fused_threads(int tx, int ty) { return tx * ty / 4 > FUSED_THREADS ? tx * ty / 4 : FUSED_THREADS; }
 
 template <typename T>
 struct Vec2;
@@ -75,7 +79,7 @@ __device__ __forceinline__ float sor_update(float omega, float bv, float temp, f
 
 template <typename T, int TX, int TY>
 struct FusedShape {
-  static constexpr int NT = FUSED_THREADS;
+  static constexpr int NT = fused_threads(TX, TY);
   static constexpr int RP = TX / 2 + 2;            // pairs per row: x = i0-2 .. i0+TX+1
   static constexpr int RY = TY + 4;                // rows: y = j0-2 .. j0+TY+1
   static constexpr int QY = TY + 2;                // red rows: y = j0-1 .. j0+TY
@@ -351,7 +355,7 @@ __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, cons
 }
 
 template <typename T, int TX, int TY, int MODE, int PF>
-__global__ void __launch_bounds__(FUSED_THREADS)
+__global__ void __launch_bounds__(fused_threads(TX, TY))
 k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G) {
   rb_fused_block<T, TX, TY, MODE, PF>(blockIdx.x, xo, xi, b, L, omega, G);
 }

@@ -54,7 +54,7 @@ struct Ops {
   void fused_launch(DevLevel<T>& d, T omega) {
     FusedGrid G = fused_grid(d, TX, TY);
     unsigned nb = (unsigned)(G.gx * G.gy * G.nch);
-    hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(FUSED_THREADS), 0, s, d.x2, d.x, d.b,
+    hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(fused_threads(TX, TY)), 0, s, d.x2, d.x, d.b,
                        d.L(), omega, G);
   }
 
@@ -66,7 +66,9 @@ struct Ops {
 
   template <int TX>
   void fused_tx(DevLevel<T>& d, T omega, int mode, int ty) {
-    if (ty == 16) {
+    if (ty == 32) {
+      fused_mode<TX, 32, 1>(d, omega, mode);
+    } else if (ty == 16) {
       if (prefetch == 2) fused_mode<TX, 16, 2>(d, omega, mode);
       else fused_mode<TX, 16, 1>(d, omega, mode);
     } else {
@@ -79,7 +81,7 @@ struct Ops {
   // in FP64 and 128 in FP32 (the same bytes per row) unless set
   int tile_x = 0;
   int tx_used() const { return (sizeof(T) == 4 && tile_x == 64) ? 64 : FTX; }
-  int ty_used() const { return tile_y ? tile_y : (sizeof(T) == 8 ? 16 : 8); }
+  int ty_used() const { return tile_y ? tile_y : 16; }
 
   // One fused red+black iteration x -> x2, then swap.  mode 0 plain, 1 zero
   // start (x taken as 0).  prefetch: planes of look-ahead of the global loads.

@@ -122,12 +122,12 @@ def run_both(dims, lens, stretch, nlevel, nu, omega, ncycle=60, smoother=0, seed
 
 
 SMOOTHER_PATHS = [dict(fused_smoother=1, tile_y=8), dict(fused_smoother=1, tile_y=16), dict(fused_smoother=2, tile_y=8),
-                  dict(fused_smoother=2, tile_y=16), dict(fused_smoother=0),
+                  dict(fused_smoother=2, tile_y=16), dict(fused_smoother=1, tile_y=32), dict(fused_smoother=0),
                   dict(fused_smoother=1, coarse_switch=0), dict(fused_smoother=1, coarse_switch=100000)]
 
 
-@pytest.mark.parametrize("opts", SMOOTHER_PATHS, ids=["fused8", "fused16", "fused8-pf2", "fused16-pf2", "colour",
-                                                      "nopersist", "persist-big"])
+@pytest.mark.parametrize("opts", SMOOTHER_PATHS, ids=["fused8", "fused16", "fused8-pf2", "fused16-pf2", "fused32",
+                                                      "colour", "nopersist", "persist-big"])
 @pytest.mark.parametrize("dims,lens,stretch,nlevel,nu,omega", VCASES)
 def test_vcycle_fp64_bitwise(dims, lens, stretch, nlevel, nu, omega, opts):
     (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both(dims, lens, stretch, nlevel, nu, omega, **opts)
@@ -147,9 +147,9 @@ def test_vcycle_cg_smoother():
     assert np.max(np.abs(xg - xo)) <= 1e-10 * np.abs(xo).max()
 
 
-@pytest.mark.parametrize("tx", [0, 64])
+@pytest.mark.parametrize("tx,ty", [(0, 0), (64, 0), (128, 32), (64, 32)])
 @pytest.mark.parametrize("dims,lens", [((64, 64, 64), (1, 1, 1)), ((32, 32, 32), (1, 1, 2))])
-def test_mixed_precision_vs_fp64_oracle(dims, lens, tx):
+def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty):
     g = O.Grid(*dims, l=lens)
     A = g.matrix()
     b = -O.uniform_field(*dims, 7)
@@ -157,7 +157,7 @@ def test_mixed_precision_vs_fp64_oracle(dims, lens, tx):
     mo = O.Multigrid(g, nl, 80)
     xo, ro = O.padded(*dims), O.padded(*dims)
     mo.solve_vcycle(xo, ro, A, b, 2, 1e-10, 1.0)
-    mg = Multigrid(sdm_from(g), nl, 80, precision=1, inner_cycles=4, tile_x=tx)
+    mg = Multigrid(sdm_from(g), nl, 80, precision=1, inner_cycles=4, tile_x=tx, tile_y=ty)
     xg, rg = O.padded(*dims), O.padded(*dims)
     res = mg.solve_vcycle(xg, rg, None, b, 2, 1e-10, 1.0)
     assert res.converged and res.rho < 1e-10
