@@ -30,7 +30,12 @@
 // whole z-slabs and the tile rings are re-read from its own L2.
 //
 // MODE 0: plain.  MODE 1: x_in = 0 (the zeroed coarse x of
-// src/multigrid.f90:1937; nothing is read).
+// src/multigrid.f90:1937; nothing is read).  MODE 2: plain, and each block
+// also writes the partial sum of (b - A x_in)^2 over its tile cells (the
+// convergence norm of src/multigrid.f90:1970-1971 for the x the iteration
+// starts from): the residual of plane m is evaluated in the red phase of
+// step m, when x_in planes m-1 .. m+1 are in LDS, with mv_mul's expression
+// (src/poisson_matrix_operator.f90:150-156).
 #pragma once
 #include <type_traits>
 
@@ -98,9 +103,9 @@ struct FusedShape {
 // chunk start k0 is odd (even chunk length), so each plane step of the
 // two-step unrolled loop knows its plane parity at compile time.
 template <typename T, int TX, int TY, int MODE, int PF>
-__device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b,
-                                              const Lvl<T>& L, T omega, int i0, int j0, int k0, int k1,
-                                              T* __restrict__ sx, T* __restrict__ sr) {
+__device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __restrict__ xi,
+                                                const T* __restrict__ b, const Lvl<T>& L, T omega, int i0, int j0,
+                                                int k0, int k1, T* __restrict__ sx, T* __restrict__ sr) {
   using S = FusedShape<T, TX, TY>;
   constexpr int NT = S::NT, RP = S::RP, RY = S::RY, QY = S::QY, TP = S::TP;
   constexpr int NXL = S::NXL, NRED = S::NRED, NBLK = S::NBLK, PLANE = S::PLANE;
@@ -147,6 +152,7 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
   bool b_on[NBLK], b_odd[NBLK], b_in[NBLK][2];
   T b_c1[NBLK][2], b_c2[NBLK][2], b_s4[NBLK][2], b_c3[NBLK], b_c4[NBLK];
   int b_row[NBLK], b_ow[NBLK][2], b_xc[NBLK][2], b_bo[NBLK][2], b_so[NBLK];
+  int b_xw[MODE == 2 ? NBLK : 1][2];        // MODE 2: x_in index of the west neighbour of b_xc
 #pragma unroll
   for (int q = 0; q < NBLK; q++) {
     int e = min(t + q * NT, TY * TP - 1);
@@ -174,6 +180,7 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
       b_ow[q][z] = b_row[q] - (bfirst ? 1 : 0);   // red x neighbours: pairs p-1,p or p,p+1
       b_xc[q][z] = (bfirst ? 0 : SB) + (ty + 2) * RP + p;
       b_bo[q][z] = in ? (gj - 1) * (int)L.px + (ic - 1) : 0;
+      if (MODE == 2) b_xw[q][z] = bfirst ? SB + (ty + 2) * RP + p - 1 : (ty + 2) * RP + p;
     }
   }
   // ---- x_in staging: pair e of a plane (row ry, pair p) ----
@@ -283,6 +290,45 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     }
   };
 
+  // MODE 2: (b - A x_in)^2 of the tile cells of plane m (parity Z), both
+  // cells of each pair: the black cell of parity Z is b_xc[q][Z], the red one
+  // b_xc[q][1-Z] (the black cell of the other parity), each with its own
+  // coefficients; b of the pair arrives as one aligned 2-element load.
+  V2 bn[MODE == 2 ? NBLK : 1];
+  double acc = 0.0;
+  auto issue_bn = [&](int m) {
+    if (MODE != 2) return;
+    const T* base = b + (long)(min(max(m, 1), L.nz) - 1) * pxy;
+#pragma unroll
+    for (int q = 0; q < NBLK; q++) bn[q] = *reinterpret_cast<const V2*>(base + b_so[q]);
+  };
+  auto norm_plane = [&](int m, auto Zc) {
+    if (MODE != 2) return;
+    constexpr int Z = decltype(Zc)::value;
+    const T* X0 = sx + (m & 3) * PLANE;
+    const T* Xm = sx + ((m - 1) & 3) * PLANE;
+    const T* Xp = sx + ((m + 1) & 3) * PLANE;
+    const T c5 = ldc(L.cwz, m), c6 = ldc(L.cez, m);
+#pragma unroll
+    for (int q = 0; q < NBLK; q++) {
+      if (!b_on[q]) continue;
+      const bool bf = Z ? !b_odd[q] : b_odd[q];   // black is the first cell iff gj + k odd
+#pragma unroll
+      for (int w = 0; w < 2; w++) {
+        const int zz = w == 0 ? Z : 1 - Z;          // w 0: black cell, 1: red cell
+        if (!b_in[q][zz]) continue;
+        const int c = b_xc[q][zz], ow = b_xw[q][zz];
+        const T c1 = b_c1[q][zz], c2 = b_c2[q][zz], c3 = b_c3[q], c4 = b_c4[q];
+        const T c0 = -(b_s4[q][zz] + c5 + c6);
+        const T ax = c0 * X0[c] + c1 * X0[ow] + c2 * X0[ow + 1] + c3 * X0[c - RP] + c4 * X0[c + RP] + c5 * Xm[c] +
+                     c6 * Xp[c];
+        const bool first = (w == 0) == bf;
+        const T r = (first ? bn[q].x : bn[q].y) - ax;
+        acc += (double)r * (double)r;
+      }
+    }
+  };
+
   using Z0 = std::integral_constant<int, 0>;
   using Z1 = std::integral_constant<int, 1>;
   // prologue: planes k0-2 .. k0 staged; k0+1 .. k0+PF in flight (k0 is odd)
@@ -320,6 +366,10 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     lds_barrier();
     red_plane(br[B], m, std::integral_constant<int, ZM>());     // needs b(m), x planes m-1..m+1
     issue_bred(br[B], m + PF, std::integral_constant<int, (ZM + PF) & 1>());
+    if (MODE == 2) {
+      if (m >= k0 && m < k1) norm_plane(m, std::integral_constant<int, ZM>());
+      issue_bn(m + 1);
+    }
     lds_barrier();
     if (m - 1 >= k0) black_plane(bk[BB], m - 1, std::integral_constant<int, 1 - ZM>());
     if (m >= k0) issue_bblk(bk[BB], m - 1 + PF, std::integral_constant<int, (1 - ZM + PF) & 1>());
@@ -332,13 +382,14 @@ __device__ __forceinline__ void rb_fused_body(T* __restrict__ xo, const T* __res
     step(m, std::integral_constant<int, 1>());
     if (m + 1 <= k1) step(m + 1, std::integral_constant<int, 0>());
   }
+  return acc;
 }
 
 // One block of the fused iteration; bid numbers the blocks of one grid.
 template <typename T, int TX, int TY, int MODE, int PF>
 __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, const T* __restrict__ xi,
                                                const T* __restrict__ b, const Lvl<T>& L, T omega,
-                                               const FusedGrid& G) {
+                                               const FusedGrid& G, double* __restrict__ partial) {
   using S = FusedShape<T, TX, TY>;
   // x_in planes (4 slots): first cells of the pairs (i = i0-2+2p, odd),
   // then second cells (i+1); new red values (4 slots), one per pair
@@ -351,13 +402,19 @@ __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, cons
   const int j0 = 1 + (rxy / G.gx) * TY;
   const int k0 = 1 + bz * G.kchunk;              // odd: kchunk is even
   const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
-  rb_fused_body<T, TX, TY, MODE, PF>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr);
+  const double acc = rb_fused_body<T, TX, TY, MODE, PF>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr);
+  if (MODE == 2) {
+    __shared__ double sh[fused_threads(TX, TY) / 64];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[bid] = s;
+  }
 }
 
 template <typename T, int TX, int TY, int MODE, int PF>
 __global__ void __launch_bounds__(fused_threads(TX, TY))
-k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G) {
-  rb_fused_block<T, TX, TY, MODE, PF>(blockIdx.x, xo, xi, b, L, omega, G);
+k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G,
+           double* __restrict__ partial) {
+  rb_fused_block<T, TX, TY, MODE, PF>(blockIdx.x, xo, xi, b, L, omega, G, partial);
 }
 
 }  // namespace pepsmg

@@ -51,30 +51,41 @@ struct Ops {
   }
 
   template <int TX, int TY, int MODE, int PF>
-  void fused_launch(DevLevel<T>& d, T omega) {
+  void fused_launch(DevLevel<T>& d, T omega, double* partial) {
     FusedGrid G = fused_grid(d, TX, TY);
     unsigned nb = (unsigned)(G.gx * G.gy * G.nch);
     hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(fused_threads(TX, TY)), 0, s, d.x2, d.x, d.b,
-                       d.L(), omega, G);
+                       d.L(), omega, G, partial);
   }
 
   template <int TX, int TY, int PF>
-  void fused_mode(DevLevel<T>& d, T omega, int mode) {
-    if (mode == 1) fused_launch<TX, TY, 1, PF>(d, omega);
-    else fused_launch<TX, TY, 0, PF>(d, omega);
+  void fused_mode(DevLevel<T>& d, T omega, int mode, double* partial) {
+    if (mode == 1) fused_launch<TX, TY, 1, PF>(d, omega, nullptr);
+    else if constexpr (sizeof(T) == 8) {
+      if (mode == 2) fused_launch<TX, TY, 2, PF>(d, omega, partial);
+      else fused_launch<TX, TY, 0, PF>(d, omega, nullptr);
+    } else {
+      fused_launch<TX, TY, 0, PF>(d, omega, nullptr);
+    }
   }
 
   template <int TX>
-  void fused_tx(DevLevel<T>& d, T omega, int mode, int ty) {
+  void fused_tx(DevLevel<T>& d, T omega, int mode, int ty, double* partial) {
     if (ty == 32) {
-      fused_mode<TX, 32, 1>(d, omega, mode);
+      fused_mode<TX, 32, 1>(d, omega, mode, partial);
     } else if (ty == 16) {
-      if (prefetch == 2) fused_mode<TX, 16, 2>(d, omega, mode);
-      else fused_mode<TX, 16, 1>(d, omega, mode);
+      if (prefetch == 2) fused_mode<TX, 16, 2>(d, omega, mode, partial);
+      else fused_mode<TX, 16, 1>(d, omega, mode, partial);
     } else {
-      if (prefetch == 2) fused_mode<TX, 8, 2>(d, omega, mode);
-      else fused_mode<TX, 8, 1>(d, omega, mode);
+      if (prefetch == 2) fused_mode<TX, 8, 2>(d, omega, mode, partial);
+      else fused_mode<TX, 8, 1>(d, omega, mode, partial);
     }
+  }
+  // blocks of one fused launch on level d (partial sums of mode 2)
+  int fused_blocks(const DevLevel<T>& d) const {
+    int tx = tx_used(), ty = ty_used();
+    FusedGrid G = fused_grid(d, tx, ty);
+    return G.gx * G.gy * G.nch;
   }
 
   // tile sizes in use: height 16 rows in FP64 and 8 in FP32, width 64 cells
@@ -84,12 +95,14 @@ struct Ops {
   int ty_used() const { return tile_y ? tile_y : 16; }
 
   // One fused red+black iteration x -> x2, then swap.  mode 0 plain, 1 zero
-  // start (x taken as 0).  prefetch: planes of look-ahead of the global loads.
+  // start (x taken as 0), 2 (FP64) plain with the partial sums of
+  // (b - A x)^2 of the x it starts from written to partial[0 .. fused_blocks).
+  // prefetch: planes of look-ahead of the global loads.
   int prefetch = 1;
-  void fused_iter(DevLevel<T>& d, T omega, int mode) {
+  void fused_iter(DevLevel<T>& d, T omega, int mode, double* partial = nullptr) {
     int ty = ty_used();
-    if (sizeof(T) == 4 && tx_used() == 64) fused_tx<64>(d, omega, mode, ty);
-    else fused_tx<FTX>(d, omega, mode, ty);
+    if (sizeof(T) == 4 && tx_used() == 64) fused_tx<64>(d, omega, mode, ty, partial);
+    else fused_tx<FTX>(d, omega, mode, ty, partial);
     std::swap(d.x, d.x2);
   }
 

@@ -304,28 +304,62 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
   }
   DevLevel<double>& f = h.lv[0];
   double hs[2];
-  op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 0);  // :1924-1925
-  HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  double res0 = hs[0];
-  *res0_out = res0;
+  // With the fused RBGS smoother the convergence norm of a cycle's result
+  // (:1970-1971) comes out of the first pre-smoothing iteration of the next
+  // cycle, which evaluates |b - A x|^2 of the x it starts from while it
+  // smooths; when the tolerance is met that iteration's output is dropped.
+  // The initial norm (:1924-1925) comes out of cycle 1 the same way, and a
+  // separate pass is left only after the last permitted cycle.
+  const bool fold = c->opt.fused_smoother && c->opt.smoother == 0 && nu > 0 && c->ncycle > 0;
+  const int nbn = fold ? op.fused_blocks(f) : 0;
+  double res0 = 0.0;
+  if (!fold) {
+    op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 0);  // :1924-1925
+    HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    res0 = hs[0];
+  }
   int ncyc = 0;
   double rho = 1.0;
+  auto check = [&](int cyc, double r2) {   // :1971-1973
+    rho = std::sqrt(r2 / res0);
+    hist.push_back(rho);
+    ncyc = cyc;
+    if (c->opt.verbose) print_cycle(cyc, r2, res0);
+    return rho < tol;
+  };
   for (int cyc = 1; cyc <= c->ncycle; cyc++) {
-    smooth_level(c, op, h, cg, 0, nu, omega, false, false);                // :1931
+    if (fold) {
+      // first pre-smoothing iteration, with |b - A x|^2 of the incoming x
+      op.fused_iter(f, omega, 2, c->d_partial);
+      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nbn,
+                         c->d_scalars + (cyc == 1 ? 0 : 1));
+      if (cyc > 1) {
+        HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        res0 = hs[0];
+        if (check(cyc - 1, hs[1])) {
+          std::swap(f.x, f.x2);   // the solution is the iteration's input
+          break;
+        }
+      }
+      smooth_level(c, op, h, cg, 0, nu - 1, omega, false, false);        // :1931 (remaining iterations)
+    } else {
+      smooth_level(c, op, h, cg, 0, nu, omega, false, false);            // :1931
+    }
     op.residual_restrict(f, h.lv[1], f.x, f.b);                            // :1932-1933
     int* slot = (cyc - 1 < c->citers_cap) ? c->d_citers + (cyc - 1) : nullptr;
     vcycle_coarse_part<double>(c, op, h, cg, nu, omega, tol, slot);        // :1936-1962
     smooth_level(c, op, h, cg, 0, nu, omega, false, true);                 // :1965-1967
-    op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 1);  // :1970-1971
-    HIP_TRY(hipMemcpyAsync(hs, c->d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    rho = std::sqrt(hs[0] / res0);
-    hist.push_back(rho);
-    ncyc = cyc;
-    if (c->opt.verbose) print_cycle(cyc, hs[0], res0);
-    if (rho < tol) break;                                                  // :1973
+    if (!fold || cyc == c->ncycle) {
+      op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 1);  // :1970-1971
+      HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      if (fold) res0 = hs[0];
+      if (check(cyc, hs[1])) break;
+    }
   }
+  *res0_out = res0;
   if (ncyc > 0) {
     int m = std::min(ncyc, c->citers_cap);
     cit.resize(m);

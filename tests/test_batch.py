@@ -1,9 +1,11 @@
 """Batched solves (pepsmg_batch_*): every item against its own single-grid
 solve through the same C-ABI, and graph replay against direct launches.
 
-In FP64 an item's batched solve runs the same kernels in the same order as
-its single-grid solve, so solutions and residual histories are identical
-bit for bit.  In mixed precision the batch always runs inner_cycles cycles
+In FP64 an item's batched solve runs the same smoothing and transfer
+kernels in the same order as its single-grid solve, so solutions are
+identical bit for bit; the residual norms are sums of the same squares in
+another order (the single-grid solver takes them from its smoother), so
+histories agree to rounding.  In mixed precision the batch always runs inner_cycles cycles
 in the last refinement step (the single-grid solver may run fewer), so the
 check there is convergence and agreement with the FP64 solution.
 """
@@ -69,7 +71,7 @@ def test_batch_fp64_matches_single_bitwise(graph):
     ref, rref = single_solves(torch, PROBLEMS, rhss)
     for i in range(B.size):
         assert res[i].converged and res[i].cycles == rref[i].cycles, (i, res[i], rref[i])
-        assert np.array_equal(res[i].history, rref[i].history), i
+        np.testing.assert_allclose(res[i].history, rref[i].history, rtol=1e-12, atol=0)
         assert torch.equal(sols[i], ref[i]), i
     B.destroy()
 
@@ -145,7 +147,7 @@ def test_batch_per_problem_tiles_fp64_bitwise():
     torch.cuda.synchronize()
     ref, rref = single_solves(torch, PROBLEMS, rhss)
     for i in range(B.size):
-        assert np.array_equal(res[i].history, rref[i].history), i
+        np.testing.assert_allclose(res[i].history, rref[i].history, rtol=1e-12, atol=0)
         assert torch.equal(sols[i], ref[i]), i
     with pytest.raises(_lib.PepsmgError):
         B.set_tiles([32, 0, 0, 0], None)
