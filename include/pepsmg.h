@@ -59,7 +59,7 @@ typedef struct pepsmg_options {
   int32_t precision;       /* 0: FP64; 1: FP32 V-cycles inside FP64 iterative refinement */
   int32_t coarse_switch;   /* levels with at most this many cells run inside one
                               persistent workgroup (0 = off); default 4096 (16^3) */
-  int32_t inner_cycles;    /* precision 1: FP32 V-cycles per refinement step (default 3) */
+  int32_t inner_cycles;    /* precision 1: FP32 V-cycles per refinement step (default 2) */
   int32_t max_refine;      /* precision 1: cap on refinement steps (default 30) */
   int32_t fused_smoother;  /* 1 (default): single-pass out-of-place red+black kernel, loads issued one
                               plane ahead; 2: the same with loads two planes ahead; 0: two colour sweeps */
@@ -67,7 +67,10 @@ typedef struct pepsmg_options {
                               max(256, tile_x * tile_y / 4) threads; 32-row tiles load one plane ahead */
   int32_t verbose;         /* 1: print the reference's [MG] cycle lines */
   int32_t tile_x;          /* FP32 fused smoother tile width 64 or 128 cells (0: 128); FP64 uses 64 */
-  int32_t inner_nu;        /* precision 1: smoothing steps of the FP32 V-cycles (default 1; 0: maxiteration) */
+  int32_t inner_nu;        /* precision 1: smoothing steps of the FP32 V-cycles (default 2; 0: maxiteration) */
+  double inner_omega;      /* precision 1: SOR factor of the FP32 V-cycles' red/black smoothing, in (0, 2)
+                              (default 1.35; 0: the caller's omega_sor); the FP64 residual and update,
+                              which decide convergence, do not use it */
 } pepsmg_options;
 
 /* Per-solve statistics (the reference only prints these, src/multigrid.f90:1972). */

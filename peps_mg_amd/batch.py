@@ -47,7 +47,7 @@ class BatchSolver:
         for k, v in kw.items():
             if not hasattr(self.options, k):
                 raise KeyError(f"unknown option {k}")
-            setattr(self.options, k, int(v))
+            setattr(self.options, k, _lib.option_value(k, v))
         _lib.check(_lib.lib().pepsmg_batch_set_options(self.h, C.byref(self.options)))
 
     def set_tiles(self, tile_x=None, tile_y=None):

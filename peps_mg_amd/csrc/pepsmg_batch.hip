@@ -597,6 +597,8 @@ int pepsmg_batch_set_options(pepsmg_batch* B, const pepsmg_options* opt) {
     return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8, 16 or 32");
   if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
   if (opt->inner_nu < 0) return fail(PEPSMG_ERR_ARG, "inner_nu must be >= 0");
+  if (!(opt->inner_omega >= 0.0 && opt->inner_omega < 2.0))
+    return fail(PEPSMG_ERR_ARG, "inner_omega must be 0 (caller's omega) or in (0, 2)");
   if (opt->inner_cycles < 1 || opt->max_refine < 1) return fail(PEPSMG_ERR_ARG, "inner_cycles/max_refine must be >= 1");
   if (opt->coarse_switch < 1) return fail(PEPSMG_ERR_ARG, "batched solves need the persistent coarse kernel (coarse_switch > 0)");
   B->opt = *opt;
@@ -686,7 +688,8 @@ static int batch_build(pepsmg_batch* B, int nu, double tol, double omega) {
     for (int p = 0; p < 2; p++) {
       Sched& S = B->sched[p];
       const int nui = B->opt.inner_nu > 0 ? B->opt.inner_nu : nu;
-      for (int q = 0; q < B->opt.inner_cycles; q++) rec_vcycle<float>(B, S, H, nui, (float)omega, tol, q == 0);
+      const float omi = (float)(B->opt.inner_omega > 0.0 ? B->opt.inner_omega : omega);
+      for (int q = 0; q < B->opt.inner_cycles; q++) rec_vcycle<float>(B, S, H, nui, omi, tol, q == 0);
       rec_resid<2>(B, S, true);
       rec_check(B, S, tol, maxsteps, 0);
     }

@@ -307,13 +307,14 @@ void pepsmg_default_options(pepsmg_options* o) {
   o->smoother = 0;
   o->precision = 0;
   o->coarse_switch = 4096;
-  o->inner_cycles = 3;
+  o->inner_cycles = 2;
   o->max_refine = 30;
   o->fused_smoother = 1;
   o->tile_y = 0;
   o->verbose = 0;
   o->tile_x = 0;
-  o->inner_nu = 1;
+  o->inner_nu = 2;
+  o->inner_omega = 1.35;
 }
 
 int pepsmg_multigrid_create(const pepsmg_grid* grid, int32_t nlevel, int32_t ncycle, int32_t aggr_method,
@@ -363,6 +364,8 @@ int pepsmg_set_options(pepsmg_ctx* ctx, const pepsmg_options* opt) {
     return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8, 16 or 32");
   if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
   if (opt->inner_nu < 0) return fail(PEPSMG_ERR_ARG, "inner_nu must be >= 0");
+  if (!(opt->inner_omega >= 0.0 && opt->inner_omega < 2.0))
+    return fail(PEPSMG_ERR_ARG, "inner_omega must be 0 (caller's omega) or in (0, 2)");
   ctx->opt = *opt;
   if (opt->precision == 1 && ctx->h32.lv.empty()) {
     int rc = ctx->h32.build(ctx->host);

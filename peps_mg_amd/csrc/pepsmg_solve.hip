@@ -408,7 +408,9 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   *res0_out = res0;
   int ncyc = 0, nref = 0;
   double rho = 1.0;
-  float om = (float)omega;
+  // SOR factor of the FP32 inner cycles (an inner-solver setting: the FP64
+  // residual and update, which decide convergence, do not use it)
+  float om = (float)(c->opt.inner_omega > 0.0 ? c->opt.inner_omega : omega);
   // smoothing steps of the FP32 V-cycles (the FP64 residual and update do
   // not depend on them): inner_nu, or the caller's maxiteration
   const int nui = c->opt.inner_nu > 0 ? c->opt.inner_nu : nu;

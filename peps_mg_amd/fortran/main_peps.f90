@@ -62,7 +62,7 @@ program main_peps
     namelist /gpu/ smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, rhs_mode, seed, output_file, &
                    write_solution
 
-    smoother = 0; precision = 0; coarse_switch = 4096; inner_cycles = 3; max_refine = 30; verbose = 0
+    smoother = 0; precision = 0; coarse_switch = 4096; inner_cycles = 2; max_refine = 30; verbose = 0
     rhs_mode = 0; seed = 20210501_8; output_file = ''; write_solution = .false.
     alpha_x = 1.0d0; alpha_y = 1.0d0; alpha_z = 1.0d0
 

@@ -17,6 +17,7 @@ module pepsmg_c
         integer(c_int32_t) :: smoother, precision, coarse_switch, inner_cycles
         integer(c_int32_t) :: max_refine, fused_smoother, tile_y, verbose
         integer(c_int32_t) :: tile_x, inner_nu
+        real(c_double)     :: inner_omega
     end type pepsmg_options_t
 
     !> struct pepsmg_stats

@@ -56,7 +56,7 @@ class Multigrid:
         for k, v in kw.items():
             if not hasattr(self.options, k):
                 raise KeyError(f"unknown option {k}")
-            setattr(self.options, k, int(v))
+            setattr(self.options, k, _lib.option_value(k, v))
         _lib.check(_lib.lib().pepsmg_set_options(self.h, C.byref(self.options)))
 
     def _stats(self):

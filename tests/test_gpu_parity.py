@@ -147,9 +147,11 @@ def test_vcycle_cg_smoother():
     assert np.max(np.abs(xg - xo)) <= 1e-10 * np.abs(xo).max()
 
 
+@pytest.mark.parametrize("inner", [dict(), dict(inner_nu=1, inner_omega=0.0, inner_cycles=4)],
+                         ids=["default", "nu1-omega1"])
 @pytest.mark.parametrize("tx,ty", [(0, 0), (64, 0), (128, 32), (64, 32)])
 @pytest.mark.parametrize("dims,lens", [((64, 64, 64), (1, 1, 1)), ((32, 32, 32), (1, 1, 2))])
-def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty):
+def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty, inner):
     g = O.Grid(*dims, l=lens)
     A = g.matrix()
     b = -O.uniform_field(*dims, 7)
@@ -157,7 +159,7 @@ def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty):
     mo = O.Multigrid(g, nl, 80)
     xo, ro = O.padded(*dims), O.padded(*dims)
     mo.solve_vcycle(xo, ro, A, b, 2, 1e-10, 1.0)
-    mg = Multigrid(sdm_from(g), nl, 80, precision=1, inner_cycles=4, tile_x=tx, tile_y=ty)
+    mg = Multigrid(sdm_from(g), nl, 80, precision=1, tile_x=tx, tile_y=ty, **inner)
     xg, rg = O.padded(*dims), O.padded(*dims)
     res = mg.solve_vcycle(xg, rg, None, b, 2, 1e-10, 1.0)
     assert res.converged and res.rho < 1e-10
