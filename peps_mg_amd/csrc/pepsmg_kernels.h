@@ -443,6 +443,25 @@ __global__ void k_pack(T* __restrict__ dst, const double* __restrict__ src, Lvl<
   }
 }
 
+// flag = 1 if a padded (0:nx+1, 0:ny+1, 0:nz+1) array has a non-zero ghost
+// cell; one wavefront per (j, k) row: whole rows on the y and z faces, the
+// two end cells of the others.
+static __global__ void __launch_bounds__(256)
+k_ghost_check(const double* __restrict__ a, int nx, int ny, int nz, int* __restrict__ flag) {
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= (long)(ny + 2) * (nz + 2)) return;
+  const int j = (int)(row % (ny + 2)), k = (int)(row / (ny + 2));
+  const double* r = a + row * (nx + 2);
+  bool bad = false;
+  if (j == 0 || k == 0 || j == ny + 1 || k == nz + 1) {
+    for (int i = lane; i <= nx + 1; i += 64) bad |= r[i] != 0.0;
+  } else if (lane < 2) {
+    bad = r[lane == 0 ? 0 : nx + 1] != 0.0;
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
 template <typename T>
 __global__ void k_unpack(double* __restrict__ dst, const T* __restrict__ src, Lvl<T> L, int zero_ghosts) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -19,6 +19,10 @@
 //                          of (Ap, p) (src/cg_poisson_matrix.f90:206-207)
 //   k_resid_march<T,TY,4>: CG start r = b - A x stored in rout and in xnew (p),
 //                          partial sums of (r, r) (:196-204)
+//   k_refine_start:        first iterative-refinement step read straight
+//       from the caller's padded (0:n+1)^3 arrays (unaligned rows): stores x
+//       and b in the solver's layout, r as float and partial r^2 (the mixed
+//       solve needs no separate copy-in pass)
 //   k_restrict_march<T>:   residual + restriction into the next level's b
 //       (src/multigrid.f90:1932-1933, :1939-1940 with :1460-1494), for
 //       levels that halve every axis
@@ -39,7 +43,7 @@ struct MarchGrid {
 // domain, adds the FP32 correction (MIXED) and writes LDS.  Keeping every
 // select away from the loads lets the compiler wait only for the loads a
 // plane needs instead of draining the ones in flight.
-template <typename T, int TY, bool MIXED>
+template <typename T, int TY, bool MIXED, bool UNAL = false>
 struct XStager {
   static constexpr int RX = 68, RY = TY + 2, XP = RX / 2;
   static constexpr int NXL = (RY * XP + MARCH_THREADS - 1) / MARCH_THREADS;
@@ -61,7 +65,12 @@ struct XStager {
       int gic = (gi >= 1 && gi <= L.nx) ? gi : 1;
       int gjc = min(max(gj, 1), L.ny);
       long q0 = lidx(L, gic, gjc, mc);
-      pre[q] = *reinterpret_cast<const V2*>(x + q0);
+      if (UNAL) {   // rows of the caller's padded layout: pairs are 8-byte aligned
+        pre[q].x = x[q0];
+        pre[q].y = x[q0 + 1];
+      } else {
+        pre[q] = *reinterpret_cast<const V2*>(x + q0);
+      }
       if (MIXED) epre[q] = *reinterpret_cast<const float2*>(e + q0);
     }
   }
@@ -106,13 +115,17 @@ __device__ __forceinline__ T march_resid(T (*s0)[RX], T (*sm)[RX], T (*sp)[RX], 
   return bv - march_apply<T, RX>(s0, sm, sp, lx, ly, c1, c2, c3, c4, c5, c6);
 }
 
+// LI: layout of the inputs x and b (L itself except for KIND 5, where it
+// describes the caller's padded arrays); bout: KIND 5's copy of b.
 template <typename T, int TY, int KIND>
 __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__ x, const float* __restrict__ e,
                                                   const T* __restrict__ b, T* __restrict__ rout,
                                                   float* __restrict__ r32, T* __restrict__ xnew, const Lvl<T>& L,
-                                                  const MarchGrid& G, double* __restrict__ partial) {
+                                                  const MarchGrid& G, double* __restrict__ partial,
+                                                  const Lvl<T>& LI, T* __restrict__ bout) {
   constexpr bool MIXED = (KIND == 2);
-  using St = XStager<T, TY, MIXED>;
+  constexpr bool UNAL = (KIND == 5);
+  using St = XStager<T, TY, MIXED, UNAL>;
   constexpr int RX = St::RX, RY = St::RY;
   constexpr int NP = (TY * 32 + MARCH_THREADS - 1) / MARCH_THREADS;   // cell pairs per thread
   using V2 = typename Vec2<T>::type;
@@ -155,15 +168,21 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
 #pragma unroll
     for (int q = 0; q < NP; q++) {
       int gi = act[q] ? i0 + 2 * lp[q] : 1, gj = act[q] ? j0 + ly[q] : 1;
-      bpre[q] = ld_stream_v2(b + lidx(L, gi, gj, mc));
+      if (UNAL) {
+        const long o = lidx(LI, gi, gj, mc);
+        bpre[q].x = ld_stream(b + o);
+        bpre[q].y = ld_stream(b + o + 1);
+      } else {
+        bpre[q] = ld_stream_v2(b + lidx(L, gi, gj, mc));
+      }
     }
   };
 
-  st.issue(x, e, L, i0, j0, k0 - 1);
+  st.issue(x, e, LI, i0, j0, k0 - 1);
   st.stage(sx[(k0 - 1) & 3], L, i0, j0);
-  st.issue(x, e, L, i0, j0, k0);
+  st.issue(x, e, LI, i0, j0, k0);
   st.stage(sx[k0 & 3], L, i0, j0);
-  st.issue(x, e, L, i0, j0, k0 + 1);
+  st.issue(x, e, LI, i0, j0, k0 + 1);
   issue_b(k0);
   double acc = 0.0;
   // plane step; the loop below runs it twice per iteration so that the
@@ -179,7 +198,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
     V2 bv[NP];
 #pragma unroll
     for (int q = 0; q < NP; q++) bv[q] = bpre[q];
-    st.issue(x, e, L, i0, j0, m + 2);
+    st.issue(x, e, LI, i0, j0, m + 2);
     issue_b(m + 1);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
@@ -207,6 +226,17 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
       if (KIND == 4) {
         if (two) st_stream_pair(xnew + q0, ra, rb);
         else xnew[q0] = ra;
+      }
+      if (KIND == 5) {   // x and b into the solver's layout
+        if (two) {
+          st_stream_pair(xnew + q0, s0[ly[q] + 1][2 * lp[q] + 2], s0[ly[q] + 1][2 * lp[q] + 3]);
+          st_stream_pair(bout + q0, bv[q].x, bv[q].y);
+          st_stream_pair(r32 + q0, (float)ra, (float)rb);
+        } else {
+          xnew[q0] = s0[ly[q] + 1][2 * lp[q] + 2];
+          bout[q0] = bv[q].x;
+          r32[q0] = (float)ra;
+        }
       }
       if (KIND == 2) {
         if (two) {
@@ -239,7 +269,18 @@ template <typename T, int TY, int KIND>
 __global__ void __launch_bounds__(MARCH_THREADS)
 k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __restrict__ b, T* __restrict__ rout,
               float* __restrict__ r32, T* __restrict__ xnew, Lvl<T> L, MarchGrid G, double* __restrict__ partial) {
-  resid_march_block<T, TY, KIND>(blockIdx.x, x, e, b, rout, r32, xnew, L, G, partial);
+  resid_march_block<T, TY, KIND>(blockIdx.x, x, e, b, rout, r32, xnew, L, G, partial, L, (T*)nullptr);
+}
+
+// First refinement step from the caller's arrays: U is the layout of x and b
+// (pitches nx+2 and (nx+2)(ny+2), base at cell (1,1,1)).
+template <int TY>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_refine_start(const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ xout,
+               double* __restrict__ bout, float* __restrict__ r32, Lvl<double> L, Lvl<double> U, MarchGrid G,
+               double* __restrict__ partial) {
+  resid_march_block<double, TY, 5>(blockIdx.x, x, (const float*)nullptr, b, (double*)nullptr, r32, xout, L, G, partial,
+                                   U, bout);
 }
 
 // Residual + restriction for a level that halves every axis.  Fine tile

@@ -373,8 +373,11 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
 
 // FP32 V-cycles inside FP64 iterative refinement.  The fine residual
 // b - A x, the update x += e and the coarsest solve stay in FP64.
+// ux, ub: the caller's padded x and b (device), read directly by the first
+// refinement step; nullptr: x and b are already in the solver's layout.
 static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vector<double>& hist,
-                       std::vector<int>& cit, double* res0_out, int* ncyc_out, int* nref_out, double* rho_out) {
+                       std::vector<int>& cit, double* res0_out, int* ncyc_out, int* nref_out, double* rho_out,
+                       const double* ux, const double* ub) {
   Hier<double>& h = c->h64;
   Hier<float>& g = c->h32;
   Ops<float> op{c->stream};
@@ -400,8 +403,19 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   };
   double hs[1];
   // r32 = b - A x and res0, with e = 0
-  hipMemsetAsync(f32.x, 0, f32.ncell * sizeof(float), c->stream);
-  refine_update();
+  if (ux) {   // x and b come from the caller's padded arrays and are stored in the solver's layout
+    Lvl<double> U = f.L();
+    U.px = f.nx + 2;
+    U.pxy = (long)(f.nx + 2) * (f.ny + 2);
+    const long o = 1 + U.px + U.pxy;   // cell (1,1,1)
+    hipLaunchKernelGGL((k_refine_start<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, ux + o,
+                       ub + o, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
+    std::swap(f.x, f.x2);
+  } else {
+    hipMemsetAsync(f32.x, 0, f32.ncell * sizeof(float), c->stream);
+    refine_update();
+  }
   HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   double res0 = hs[0];
@@ -505,14 +519,15 @@ static int check_ghosts_host(const double* a, int nx, int ny, int nz, const char
   return PEPSMG_OK;
 }
 
-static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_stats* st) {
+static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_stats* st, const double* ux = nullptr,
+                     const double* ub = nullptr) {
   std::vector<double> hist;
   std::vector<int> cit;
   double res0 = 0, rho = 0;
   int ncyc = 0, nref = 0, rc;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   if (c->opt.precision == 1)
-    rc = solve_mixed(c, nu, tol, omega, hist, cit, &res0, &ncyc, &nref, &rho);
+    rc = solve_mixed(c, nu, tol, omega, hist, cit, &res0, &ncyc, &nref, &rho, ux, ub);
   else
     rc = solve_fp64(c, nu, tol, omega, st, hist, cit, &res0, &ncyc, &rho);
   if (rc) return rc;
@@ -549,14 +564,24 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
   DevLevel<double>& f = c->h64.lv[0];
   int rc = PEPSMG_OK;
   int flag = 0;
+  // Mixed precision: the first refinement step reads the caller's arrays and
+  // leaves x and b in the solver's layout, so only the ghost layer of sol is
+  // checked here; FP64: copy-in passes (with the same check).
+  const bool direct = c->opt.precision == 1;
   hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
-  pack(c->stream, f.x, d_sol, f, c->d_flag);
-  pack(c->stream, f.b, d_rhs, f, (int*)nullptr);
+  if (direct) {
+    const long rows = (long)(f.ny + 2) * (f.nz + 2);
+    hipLaunchKernelGGL(k_ghost_check, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, d_sol, f.nx, f.ny,
+                       f.nz, c->d_flag);
+  } else {
+    pack(c->stream, f.x, d_sol, f, c->d_flag);
+    pack(c->stream, f.b, d_rhs, f, (int*)nullptr);
+  }
   hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) rc = hip_fail(e, "pack");
   else if (flag) rc = fail(PEPSMG_ERR_GHOST, "sol has non-zero ghost cells (homogeneous Dirichlet required)");
-  if (!rc) rc = run_solve(c, maxiteration, tolerance, omega_sor, st);
+  if (!rc) rc = run_solve(c, maxiteration, tolerance, omega_sor, st, direct ? d_sol : nullptr, direct ? d_rhs : nullptr);
   if (!rc) {
     unpack(c->stream, d_sol, f.x, f, 0);
     if (d_rsd) {

@@ -171,12 +171,14 @@ def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty, inner):
     assert err < 1e-8, err
 
 
-def test_ghost_cells_rejected():
+@pytest.mark.parametrize("precision,where", [(0, (0, 3, 3)), (1, (0, 3, 3)), (1, (3, 9, 3)), (1, (3, 3, 9)),
+                                             (1, (9, 9, 9))])
+def test_ghost_cells_rejected(precision, where):
     dims = (8, 8, 8)
     g = O.Grid(*dims)
-    mg = Multigrid(sdm_from(g), 2, 10)
+    mg = Multigrid(sdm_from(g), 2, 10, precision=precision)
     x, r = O.padded(*dims), O.padded(*dims)
-    x[0, 3, 3] = 1.0
+    x[where] = 1.0
     with pytest.raises(_lib.PepsmgError) as e:
         mg.solve_vcycle(x, r, None, rand_padded(*dims), 2, 1e-10, 1.0)
     assert e.value.code == _lib.ERR_GHOST
