@@ -46,6 +46,10 @@ def parse():
     p.add_argument("--no-tune", action="store_true", help="batched workload: skip the autotuner")
     p.add_argument("--batch", action="store_true",
                    help="run the batched workload (48 cubes of mixed sizes, 8 with 4 right-hand sides) instead")
+    p.add_argument("--variants", action="store_true",
+                   help="run the FP64 GPU variants (RBGS or CG smoother with 4 iterations; coarsening to 4^3 in "
+                        "per-level kernels or below 16^3 in the persistent kernel) and mixed precision at 256^3 "
+                        "and 512^3 instead")
     p.add_argument("--kernels-only", action="store_true",
                    help="run only the fine-level smoother (FP64 and FP32) and a 1 GiB device copy, for "
                         "rocprofv3 counter passes; prints one JSON line with the event timings")
@@ -215,6 +219,54 @@ def adjoint_workload(a):
                       "fp64": out["fp64"], "mixed": out["mixed"]}))
 
 
+def variants_workload(a):
+    """Time to 1e-10 relative residual for the smoother and coarsest-level
+    variants of the FP64 GPU drop-in (red/black Gauss-Seidel or CG with 4
+    iterations per smoothing step; every level down to 4^3 in per-level
+    kernels, or levels of at most 16^3 cells inside the persistent coarse
+    kernel) and for mixed precision, at 256^3 and 512^3 (same right-hand side
+    as the default workload).  value = the default FP64 solve at 512^3."""
+    import torch
+    from peps_mg_amd import _lib, geometry
+    from peps_mg_amd.multigrid import Multigrid
+    cases = [("rbgs_fp64_persist16", dict(), 2), ("rbgs_fp64_full_coarsening", dict(coarse_switch=0), 2),
+             ("cg4_fp64_persist16", dict(smoother=1), 4), ("cg4_fp64_full_coarsening", dict(smoother=1, coarse_switch=0), 4),
+             ("rbgs_mixed", dict(precision=1), 2), ("cg4_mixed", dict(smoother=1, precision=1), 4)]
+    out = {}
+    for n in (256, 512):
+        levels = int(round(math.log2(n // 4)))
+        sdm = geometry.geometry_subdomain_create(geometry.geometry_domain_create(n, n, n, 0.0, 0.0, 0.0, 1.0, 1.0,
+                                                                                 a.lz))
+        d_b = torch.zeros((n + 2,) * 3, dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib().pepsmg_fill_rhs_device(n, n, n, C.c_void_p(d_b.data_ptr()), 20210501, -1.0,
+                                                     -3 * math.pi ** 2, _lib.dptr(sdm.xg), _lib.dptr(sdm.yg),
+                                                     _lib.dptr(sdm.zg), None))
+        d_x = torch.zeros_like(d_b)
+        stream = torch.cuda.current_stream().cuda_stream
+        for label, kw, nu in cases:
+            mg = Multigrid(sdm, levels, 200, **kw)
+            best, res = float("inf"), None
+            for _ in range(1 + max(1, a.steps // 2)):
+                d_x.zero_()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                res = mg.solve_vcycle_device(d_x.data_ptr(), 0, d_b.data_ptr(), nu, 1e-10, 1.0, stream)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            out[f"{n}^3/{label}"] = {"s": round(best, 5), "cycles": res.cycles, "refine_steps": res.refine_steps,
+                                     "converged": bool(res.converged), "rho": res.rho}
+            log(f"[bench] {n}^3 {label}: {best * 1e3:.2f} ms, {res.cycles} cycles")
+            mg.destroy()
+        del d_x, d_b
+        torch.cuda.empty_cache()
+    print(json.dumps({"metric": "time to 1e-10 rel. residual, FP64 GPU variants and mixed precision",
+                      "value": out["512^3/rbgs_fp64_persist16"]["s"], "unit": "s", "higher_is_better": False,
+                      "n_gpus": 1, "data": "synthetic",
+                      "config": {"workload": f"Poisson 7-point, 256^3 and 512^3 cells, unit cube x (1,1,{a.lz:g}), "
+                                             "homogeneous Dirichlet, V-cycles to 4^3, tol 1e-10, x0=0"},
+                      "variants": out}))
+
+
 def batch_workload(a):
     """48 independent Dirichlet Poisson problems on unit cubes with edges
     drawn from {32, 48, 64, 96, 128, 192} (seeded), each with its own
@@ -344,6 +396,9 @@ def main():
         return
     if a.batch:
         batch_workload(a)
+        return
+    if a.variants:
+        variants_workload(a)
         return
     import numpy as np
     import torch
