@@ -334,7 +334,7 @@ static void rec_smooth(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int 
     if (l >= B->items[i].s) continue;
     int tx = B->items[i].tile_x ? B->items[i].tile_x : B->opt.tile_x;
     int ty = B->items[i].tile_y ? B->items[i].tile_y : B->opt.tile_y;
-    if (!ty) ty = 16;
+    if (!ty) ty = sizeof(T) == 8 ? 16 : 32;
     int wx = (sizeof(T) == 4 && tx != 64) ? 1 : 0;
     g[wx][ty == 32 ? 2 : ty == 16 ? 1 : 0].push_back(i);
   }

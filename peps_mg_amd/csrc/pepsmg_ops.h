@@ -134,11 +134,11 @@ struct Ops {
     return G.gx * G.gy * G.nch;
   }
 
-  // tile sizes in use: height 16 rows in FP64 and 8 in FP32, width 64 cells
+  // tile sizes in use: height 16 rows in FP64 and 32 in FP32, width 64 cells
   // in FP64 and 128 in FP32 (the same bytes per row) unless set
   int tile_x = 0;
   int tx_used() const { return (sizeof(T) == 4 && tile_x == 64) ? 64 : FTX; }
-  int ty_used() const { return tile_y ? tile_y : 16; }
+  int ty_used() const { return tile_y ? tile_y : (sizeof(T) == 8 ? 16 : 32); }
 
   // One fused red+black iteration x -> x2, then swap.  mode 0 plain, 1 zero
   // start (x taken as 0), 2 (FP64) plain with the partial sums of
