@@ -117,7 +117,8 @@ int pepsmg_multigrid_solve_vcycle(pepsmg_ctx *ctx, double *sol, double *rsd, con
                                   double omega_sor, pepsmg_stats *stats);
 
 /* Same on device arrays (padded layout) on the given HIP stream (NULL: the
- * context's own stream).  rsd may be NULL. */
+ * context's own stream, ordered after the work already queued on the default
+ * stream).  Returns after the stream has finished.  rsd may be NULL. */
 int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx *ctx, double *d_sol, double *d_rsd,
                                          const double *d_rhs, int32_t maxiteration,
                                          double tolerance, double omega_sor, void *stream,
@@ -220,7 +221,8 @@ int pepsmg_batch_set_tiles(pepsmg_batch *batch, const int32_t *tile_x, const int
 /* Solves every item from its initial guess.  d_sol[i] / d_rhs[i]: device
  * arrays of item i in the padded layout (ghosts zero), overwritten /
  * read.  use_graph 1 replays the captured graph, 0 launches the same
- * sequence directly.  stats: NULL or an array of batch_size entries
+ * sequence directly.  stream NULL: the batch's own stream, ordered after the
+ * work already queued on the default stream.  stats: NULL or an array of batch_size entries
  * (history as for a single solve; device_ms is the time of the whole batch). */
 int pepsmg_batch_solve_device(pepsmg_batch *batch, double *const *d_sol, const double *const *d_rhs,
                               int32_t maxiteration, double tolerance, double omega_sor, int32_t use_graph,

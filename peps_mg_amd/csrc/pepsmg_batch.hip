@@ -245,6 +245,7 @@ struct pepsmg_batch {
   double key_tol = -1, key_omega = 0;
   pepsmg_options key_opt;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_entry = nullptr;   // orders the own stream after the caller's default-stream work
 
   void drop_schedule() {
     for (int p = 0; p < 2; p++) {
@@ -278,6 +279,8 @@ struct pepsmg_batch {
     d_out = d_res0 = d_hist = d_partial = nullptr;
     d_arena = nullptr;
     if (ev0) hipEventDestroy(ev0);
+    if (ev_entry) hipEventDestroy(ev_entry);
+    ev_entry = nullptr;
     if (ev1) hipEventDestroy(ev1);
     ev0 = ev1 = nullptr;
     if (stream) hipStreamDestroy(stream);
@@ -583,6 +586,7 @@ int pepsmg_batch_create(int32_t nprob, const pepsmg_grid* grids, const int32_t* 
   if ((rc = chk(hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking), "hipStreamCreate"))) return rc;
   if ((rc = chk(hipEventCreate(&B->ev0), "hipEventCreate"))) return rc;
   if ((rc = chk(hipEventCreate(&B->ev1), "hipEventCreate"))) return rc;
+  if ((rc = chk(hipEventCreateWithFlags(&B->ev_entry, hipEventDisableTiming), "hipEventCreate"))) return rc;
   *out = B;
   return PEPSMG_OK;
 }
@@ -753,6 +757,10 @@ int pepsmg_batch_solve_device(pepsmg_batch* B, double* const* d_sol, const doubl
   for (int i = 0; i < B->nitems; i++)
     if (!d_sol[i] || !d_rhs[i]) return fail(PEPSMG_ERR_ARG, "item %d: NULL array", i);
   hipStream_t st = stream ? (hipStream_t)stream : B->stream;
+  if (!stream) {   // the own stream does not synchronize with the default stream: order it after the caller's work there
+    HIP_TRY(hipEventRecord(B->ev_entry, (hipStream_t)0));
+    HIP_TRY(hipStreamWaitEvent(st, B->ev_entry, 0));
+  }
   const bool mixed = B->opt.precision == 1;
   const int n = B->nitems;
   // pointer state at the start of a solve: x in lv[0].x for every item

@@ -107,7 +107,7 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
                                                 const T* __restrict__ b, const Lvl<T>& L, T omega, int i0, int j0,
                                                 int k0, int k1, T* __restrict__ sx, T* __restrict__ sr) {
   using S = FusedShape<T, TX, TY>;
-  constexpr int NT = S::NT, RP = S::RP, RY = S::RY, QY = S::QY, TP = S::TP;
+  constexpr int NT = S::NT, RP = S::RP, QY = S::QY, TP = S::TP;
   constexpr int NXL = S::NXL, NRED = S::NRED, NBLK = S::NBLK, PLANE = S::PLANE;
   constexpr int SB = 4 * PLANE;           // second-cell slots follow the four first-cell slots
   constexpr int QP = QY * RP;             // one red plane

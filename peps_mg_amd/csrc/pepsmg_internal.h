@@ -124,6 +124,11 @@ struct pepsmg_ctx {
   size_t pad_elems = 0;
   double* d_pad2 = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // convergence read-back: pinned host copy of d_scalars[0..7] and the event
+  // after its copy, so the host can wait for the norm while later work runs
+  double* h_scalars = nullptr;
+  hipEvent_t ev_norm = nullptr;
+  hipEvent_t ev_entry = nullptr;   // orders the own stream after the caller's default-stream work
 
   int alloc_scratch();
   void release();

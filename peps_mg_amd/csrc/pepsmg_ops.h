@@ -89,12 +89,15 @@ struct Ops {
     return resident((const void*)k_rb_fused<T, TX, TY, 0, PF>, fused_threads(TX, TY));
   }
 
+  // output buffer of the next fused launch (nullptr: d.x2)
+  T* fused_out = nullptr;
+
   template <int TX, int TY, int MODE, int PF>
   void fused_launch(DevLevel<T>& d, T omega, double* partial) {
     FusedGrid G = fused_grid(d, TX, TY, fused_resident<TX, TY, PF>());
     unsigned nb = (unsigned)(G.gx * G.gy * G.nch);
-    hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(fused_threads(TX, TY)), 0, s, d.x2, d.x, d.b,
-                       d.L(), omega, G, partial);
+    hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(fused_threads(TX, TY)), 0, s,
+                       fused_out ? fused_out : d.x2, d.x, d.b, d.L(), omega, G, partial);
   }
 
   template <int TX, int TY, int PF>
@@ -150,6 +153,14 @@ struct Ops {
     if (sizeof(T) == 4 && tx_used() == 64) fused_tx<64>(d, omega, mode, ty, partial);
     else fused_tx<FTX>(d, omega, mode, ty, partial);
     std::swap(d.x, d.x2);
+  }
+  // one fused iteration (mode 0) from d.x into out; no swap
+  void fused_iter_into(DevLevel<T>& d, T omega, T* out) {
+    fused_out = out;
+    int ty = ty_used();
+    if (sizeof(T) == 4 && tx_used() == 64) fused_tx<64>(d, omega, 0, ty, nullptr);
+    else fused_tx<FTX>(d, omega, 0, ty, nullptr);
+    fused_out = nullptr;
   }
 
   void smooth_colour(DevLevel<T>& d, const T* b, T* x, T omega, int colour, bool zero) {

@@ -166,6 +166,9 @@ int pepsmg_ctx::alloc_scratch() {
   HIP_TRY(hipMalloc(&d_citers, citers_cap * sizeof(int)));
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
+  HIP_TRY(hipEventCreateWithFlags(&ev_norm, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&ev_entry, hipEventDisableTiming));
+  HIP_TRY(hipHostMalloc((void**)&h_scalars, 8 * sizeof(double), hipHostMallocDefault));
   return PEPSMG_OK;
 }
 
@@ -184,7 +187,11 @@ void pepsmg_ctx::release() {
   d_flag = d_citers = nullptr;
   if (ev0) hipEventDestroy(ev0);
   if (ev1) hipEventDestroy(ev1);
-  ev0 = ev1 = nullptr;
+  if (ev_norm) hipEventDestroy(ev_norm);
+  if (ev_entry) hipEventDestroy(ev_entry);
+  ev0 = ev1 = ev_norm = ev_entry = nullptr;
+  if (h_scalars) hipHostFree(h_scalars);
+  h_scalars = nullptr;
   if (stream) hipStreamDestroy(stream);
   stream = nullptr;
 }
@@ -321,6 +328,13 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
   }
   int ncyc = 0;
   double rho = 1.0;
+  // the per-cycle factor of the last two cycles predicts that the next
+  // check meets the tolerance (factor-2 margin)
+  auto likely_converged = [&]() {
+    const size_t n = hist.size();
+    if (n < 2 || !(hist[n - 2] > 0.0)) return false;
+    return hist[n - 1] * (hist[n - 1] / hist[n - 2]) < 2.0 * tol;
+  };
   auto check = [&](int cyc, double r2) {   // :1971-1973
     rho = std::sqrt(r2 / res0);
     hist.push_back(rho);
@@ -334,16 +348,28 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
       op.fused_iter(f, omega, 2, c->d_partial);
       hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nbn,
                          c->d_scalars + (cyc == 1 ? 0 : 1));
+      int pre_done = 1;
       if (cyc > 1) {
-        HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        res0 = hs[0];
-        if (check(cyc - 1, hs[1])) {
+        HIP_TRY(hipMemcpyAsync(c->h_scalars, c->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipEventRecord(c->ev_norm, c->stream));
+        // While the host waits for the norm, the GPU runs the second
+        // iteration into the spare buffer f.r; the first iteration's input,
+        // the answer if the tolerance is met, stays in f.x2.  Skipped when
+        // the check is predicted to stop the solve.
+        const bool ahead = nu >= 2 && !likely_converged();
+        if (ahead) op.fused_iter_into(f, omega, f.r);
+        HIP_TRY(hipEventSynchronize(c->ev_norm));
+        res0 = c->h_scalars[0];
+        if (check(cyc - 1, c->h_scalars[1])) {
           std::swap(f.x, f.x2);   // the solution is the iteration's input
           break;
         }
+        if (ahead) {
+          std::swap(f.x, f.r);   // the iterate after two iterations; f.r is spare again
+          pre_done = 2;
+        }
       }
-      smooth_level(c, op, h, cg, 0, nu - 1, omega, false, false);        // :1931 (remaining iterations)
+      smooth_level(c, op, h, cg, 0, nu - pre_done, omega, false, false);  // :1931 (remaining iterations)
     } else {
       smooth_level(c, op, h, cg, 0, nu, omega, false, false);            // :1931
     }
@@ -401,7 +427,6 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
     std::swap(f.x, f.x2);
   };
-  double hs[1];
   // r32 = b - A x and res0, with e = 0
   if (ux) {   // x and b come from the caller's padded arrays and are stored in the solver's layout
     Lvl<double> U = f.L();
@@ -416,18 +441,30 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     hipMemsetAsync(f32.x, 0, f32.ncell * sizeof(float), c->stream);
     refine_update();
   }
-  HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  double res0 = hs[0];
-  *res0_out = res0;
-  int ncyc = 0, nref = 0;
-  double rho = 1.0;
   // SOR factor of the FP32 inner cycles (an inner-solver setting: the FP64
   // residual and update, which decide convergence, do not use it)
   float om = (float)(c->opt.inner_omega > 0.0 ? c->opt.inner_omega : omega);
   // smoothing steps of the FP32 V-cycles (the FP64 residual and update do
   // not depend on them): inner_nu, or the caller's maxiteration
   const int nui = c->opt.inner_nu > 0 ? c->opt.inner_nu : nu;
+  // The host reads each norm while the GPU already runs the next step's
+  // first pre-smoothing (it needs only r32, and touches no FP64 array), so
+  // the read-back leaves no gap; skipped when the step is predicted to stop.
+  bool ahead = false;
+  auto read_norm = [&](bool run_ahead) -> int {
+    HIP_TRY(hipMemcpyAsync(c->h_scalars, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(c->ev_norm, c->stream));
+    ahead = run_ahead;
+    if (ahead) smooth_level(c, op, g, cg, 0, nui, om, true, false);
+    HIP_TRY(hipEventSynchronize(c->ev_norm));
+    return PEPSMG_OK;
+  };
+  int rc = read_norm(c->ncycle > 0 && c->opt.max_refine > 0);
+  if (rc) return rc;
+  double res0 = c->h_scalars[0];
+  *res0_out = res0;
+  int ncyc = 0, nref = 0;
+  double rho = 1.0;
   // Per-V-cycle reduction factor of the previous step.  A step that is
   // predicted (with a factor-2 margin) to reach the tolerance in fewer than
   // inner_cycles cycles runs only those: typically the last step, which needs
@@ -448,19 +485,22 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     double rho_prev = rho;
     for (int q = 0; q < nin; q++) {
       int* slot = (ncyc < c->citers_cap) ? c->d_citers + ncyc : nullptr;
-      smooth_level(c, op, g, cg, 0, nui, om, q == 0, false);
+      if (!(q == 0 && ahead)) smooth_level(c, op, g, cg, 0, nui, om, q == 0, false);
       op.residual_restrict(f32, g.lv[1], f32.x, f32.b);
       vcycle_coarse_part<float>(c, op, g, cg, nui, om, tol, slot);
       smooth_level(c, op, g, cg, 0, nui, om, false, true);
       ncyc++;
     }
     refine_update();
-    HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    rho = std::sqrt(hs[0] / res0);
+    // predicted to stop here: the previous per-cycle factor over this step's cycles, factor-2 margin
+    const bool stop_likely = fcyc > 0.0 && rho_prev * std::pow(fcyc, nin) < 2.0 * tol;
+    rc = read_norm(!stop_likely && s < c->opt.max_refine && ncyc < c->ncycle);
+    if (rc) return rc;
+    const double r2 = c->h_scalars[0];
+    rho = std::sqrt(r2 / res0);
     hist.push_back(rho);
     nref = s;
-    if (c->opt.verbose) print_cycle(s, hs[0], res0);
+    if (c->opt.verbose) print_cycle(s, r2, res0);
     if (rho < tol) break;
     if (rho > 0.0 && rho_prev > 0.0) fcyc = std::pow(rho / rho_prev, 1.0 / nin);
   }
@@ -560,7 +600,15 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
   if (!c || !d_sol || !d_rhs) return fail(PEPSMG_ERR_ARG, "NULL argument");
   if (maxiteration < 0) return fail(PEPSMG_ERR_ARG, "maxiteration < 0");
   hipStream_t saved = c->stream;
-  if (stream) c->stream = (hipStream_t)stream;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    // the context's own stream does not synchronize with the default
+    // stream: order it after the work the caller queued there (for example
+    // the writes of sol and rhs)
+    HIP_TRY(hipEventRecord(c->ev_entry, (hipStream_t)0));
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_entry, 0));
+  }
   DevLevel<double>& f = c->h64.lv[0];
   int rc = PEPSMG_OK;
   int flag = 0;
@@ -577,11 +625,19 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
     pack(c->stream, f.x, d_sol, f, c->d_flag);
     pack(c->stream, f.b, d_rhs, f, (int*)nullptr);
   }
-  hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-  hipError_t e = hipStreamSynchronize(c->stream);
+  // the flag is read back with the solve's first norm (the solve never
+  // reads ghost cells and writes nothing of the caller's before the check)
+  int* h_flag = reinterpret_cast<int*>(c->h_scalars + 7);
+  *h_flag = 0;
+  hipError_t e = hipMemcpyAsync(h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e != hipSuccess) rc = hip_fail(e, "pack");
-  else if (flag) rc = fail(PEPSMG_ERR_GHOST, "sol has non-zero ghost cells (homogeneous Dirichlet required)");
   if (!rc) rc = run_solve(c, maxiteration, tolerance, omega_sor, st, direct ? d_sol : nullptr, direct ? d_rhs : nullptr);
+  if (!rc) {
+    e = hipStreamSynchronize(c->stream);
+    flag = *h_flag;
+    if (e != hipSuccess) rc = hip_fail(e, "pack");
+    else if (flag) rc = fail(PEPSMG_ERR_GHOST, "sol has non-zero ghost cells (homogeneous Dirichlet required)");
+  }
   if (!rc) {
     unpack(c->stream, d_sol, f.x, f, 0);
     if (d_rsd) {
@@ -659,7 +715,7 @@ int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double
     n = (double)d.cells();
     if (fused)
       std::snprintf(name, sizeof name, "k_rb_fused<%s, %d, %d, 0, %d>", tname, op.tx_used(), op.ty_used(),
-                    op.prefetch);
+                    op.ty_used() == 32 ? 1 : op.prefetch);   // 32-row tiles load one plane ahead
     else
       std::snprintf(name, sizeof name, "k_rb_colour<%s, false>", tname);
     return PEPSMG_OK;

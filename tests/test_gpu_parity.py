@@ -259,3 +259,31 @@ def test_full_size_256_properties(precision):
     r = d_b[1:-1, 1:-1, 1:-1] - ax
     rho = float(torch.sqrt((r * r).sum() / res.res0))
     assert rho < 1.05e-10 and abs(rho - res.rho) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_solve_orders_after_default_stream_work(precision):
+    """A device solve with stream NULL (the library's own stream) waits for
+    work the caller queued on the default stream: here a long matrix product
+    followed by zeroing sol; the result equals a solve from a synchronized
+    zero start, bit for bit, on every repetition."""
+    import torch
+    n = 64
+    g = O.Grid(n, n, n)
+    sdm = sdm_from(g)
+    mg = Multigrid(sdm, 4, 60, precision=precision)
+    b = torch.from_numpy(-O.uniform_field(n, n, n, 11)).cuda()
+    ref = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    mg.solve_vcycle_device(ref.data_ptr(), 0, b.data_ptr(), 2, 1e-10, 1.0)
+    big = torch.randn(6144, 6144, device="cuda")
+    x = torch.zeros_like(b)
+    for _ in range(3):
+        x.fill_(1.0)
+        torch.cuda.synchronize()
+        y = big @ big                    # keeps the default stream busy for milliseconds
+        x.zero_()
+        mg.solve_vcycle_device(x.data_ptr(), 0, b.data_ptr(), 2, 1e-10, 1.0)
+        assert torch.equal(x, ref)
+        del y
+    mg.destroy()
