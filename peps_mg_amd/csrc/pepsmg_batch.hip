@@ -108,7 +108,7 @@ kb_restrict(const RestrictItem<T>* __restrict__ items, int n, const int* __restr
   const int q = batch_find(items, n, blockIdx.x);
   const RestrictItem<T>& it = items[q];
   if (!active[it.item]) return;
-  restrict_march_block<T>(blockIdx.x - it.block0, it.bc, it.x, it.b, it.F, it.C, it.W, it.G);
+  restrict_march_block<T, 64>(blockIdx.x - it.block0, it.bc, it.x, it.b, it.F, it.C, it.W, it.G);
 }
 
 template <typename T>
@@ -117,7 +117,7 @@ kb_prolong(const ProlongItem<T>* __restrict__ items, int n, const int* __restric
   const int q = batch_find(items, n, blockIdx.x);
   const ProlongItem<T>& it = items[q];
   if (!active[it.item]) return;
-  prolong_march_block<T>(blockIdx.x - it.block0, it.xf, it.xc, it.F, it.C, it.W, it.G);
+  prolong_march_block<T, 64>(blockIdx.x - it.block0, it.xf, it.xc, it.F, it.C, it.W, it.G);
 }
 
 template <int KIND>

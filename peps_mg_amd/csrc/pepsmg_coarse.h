@@ -9,18 +9,20 @@ namespace pepsmg {
 // x_f += P x_c for a level that halves every axis
 // (src/multigrid.f90:1958-1959 / :1965-1966 with
 // multigrid_prolongation_linear_on_nonuniform_grid :1676-1754).  A block
-// owns a 64 x 16 fine tile (a 32 x 8 coarse tile); coarse planes with a
+// owns a TX x 16 fine tile (a TX/2 x 8 coarse tile; TX = 64, or 128 for
+// FP32, with 4 TX threads); coarse planes with a
 // one-cell ring are staged in LDS (4 slots), fine x planes stream through
 // registers with the next plane's loads in flight.  Per axis the fine cell f
 // takes its parent c with weight ws and the neighbour nb with weight wn
 // (f odd: dxp(f), dxp(f+1), c-1; f even: dxn(f), dxn(f-1), c+1); the eight
 // terms are summed in the reference order.
-template <typename T>
+template <typename T, int TX>
 __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf, const T* __restrict__ xc,
                                                     const Lvl<T>& F, const Lvl<T>& Cc, const ProlongW<T>& W,
                                                     const MarchGrid& G) {
-  constexpr int CX = 34, CY = 10;                    // coarse tile 32 x 8 + ring 1
-  constexpr int NCL = (CX * CY + MARCH_THREADS - 1) / MARCH_THREADS;
+  constexpr int TP = TX / 2, NT = march_threads<TX>();
+  constexpr int CX = TP + 2, CY = 10;                // coarse tile TX/2 x 8 + ring 1
+  constexpr int NCL = (CX * CY + NT - 1) / NT;
   constexpr int NP = 2;                              // fine pairs per thread
   using V2 = typename Vec2<T>::type;
   __shared__ T sc[4][CY][CX];
@@ -28,25 +30,25 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
   const int t = threadIdx.x;
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
-  const int i0 = 1 + (rxy % G.gx) * 64;              // fine tile origin
+  const int i0 = 1 + (rxy % G.gx) * TX;              // fine tile origin
   const int j0 = 1 + (rxy / G.gx) * 16;
   const int ic0 = (i0 + 1) / 2, jc0 = (j0 + 1) / 2;  // coarse tile origin
   const int kc0 = 1 + bz * G.kchunk;                 // coarse planes of this chunk
   const int kc1 = min(kc0 + G.kchunk, Cc.nz + 1);
 
-  // fine pairs: rows ly = t/32 + 8q, pair lp = t%32 -> cells i0+2lp (odd), i0+2lp+1
+  // fine pairs: rows ly = t/TP + 8q, pair lp = t%TP -> cells i0+2lp (odd), i0+2lp+1
   int ly[NP];
   bool act[NP], two[NP];
   T wsa[NP], wna[NP], wsb[NP], wnb[NP], wsy[NP], wny[NP];
   int cjr[NP], njr[NP];   // coarse row (LDS) of parent / neighbour in y
-  const int lp = t % 32;
+  const int lp = t % TP;
   const int ia = i0 + 2 * lp;
   // x: cell ia (odd): parent ic, neighbour ic-1; cell ia+1 (even): parent ic, neighbour ic+1
   const int ic = (ia + 1) / 2;
   const int lcx = ic - ic0 + 1;                      // LDS column of the parent
 #pragma unroll
   for (int q = 0; q < NP; q++) {
-    ly[q] = t / 32 + 8 * q;
+    ly[q] = t / TP + 8 * q;
     int gj = j0 + ly[q];
     act[q] = gj <= F.ny && ia <= F.nx;
     two[q] = act[q] && ia + 1 <= F.nx;
@@ -72,7 +74,7 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
     const int kcc = min(max(kc, 1), Cc.nz);
 #pragma unroll
     for (int q = 0; q < NCL; q++) {
-      int e = min(t + q * MARCH_THREADS, CX * CY - 1);
+      int e = min(t + q * NT, CX * CY - 1);
       int cy = e / CX, cx = e - cy * CX;
       int gi = min(max(ic0 - 1 + cx, 1), Cc.nx), gj = min(max(jc0 - 1 + cy, 1), Cc.ny);
       cpre[q] = xc[lidx(Cc, gi, gj, kcc)];
@@ -82,7 +84,7 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
     const bool vz = cq >= 1 && cq <= Cc.nz;
 #pragma unroll
     for (int q = 0; q < NCL; q++) {
-      int e = t + q * MARCH_THREADS;
+      int e = t + q * NT;
       if (e < CX * CY) {
         int cy = e / CX, cx = e - cy * CX;
         int gi = ic0 - 1 + cx, gj = jc0 - 1 + cy;
@@ -166,10 +168,10 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
   if (kc < kc1) step(kc);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(MARCH_THREADS)
+template <typename T, int TX>
+__global__ void __launch_bounds__(march_threads<TX>())
 k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> Cc, ProlongW<T> W, MarchGrid G) {
-  prolong_march_block<T>(blockIdx.x, xf, xc, F, Cc, W, G);
+  prolong_march_block<T, TX>(blockIdx.x, xf, xc, F, Cc, W, G);
 }
 
 // ---------------------------------------------------------------------------

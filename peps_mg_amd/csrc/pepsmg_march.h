@@ -37,16 +37,16 @@ struct MarchGrid {
   int gx, gy, nch, kchunk;
 };
 
-// Stages x planes (4 slots) of a 64 x TY tile: x = i0-2 .. i0+65, y = j0-1 .. j0+TY.
+// Stages x planes (4 slots) of a TX x TY tile: x = i0-2 .. i0+TX+1, y = j0-1 .. j0+TY.
 // issue() loads one plane into registers from clamped in-bounds addresses
 // with no branch around the loads; stage() zeroes what lies outside the
 // domain, adds the FP32 correction (MIXED) and writes LDS.  Keeping every
 // select away from the loads lets the compiler wait only for the loads a
 // plane needs instead of draining the ones in flight.
-template <typename T, int TY, bool MIXED, bool UNAL = false>
+template <typename T, int TY, bool MIXED, bool UNAL = false, int TX = 64, int NT = MARCH_THREADS>
 struct XStager {
-  static constexpr int RX = 68, RY = TY + 2, XP = RX / 2;
-  static constexpr int NXL = (RY * XP + MARCH_THREADS - 1) / MARCH_THREADS;
+  static constexpr int RX = TX + 4, RY = TY + 2, XP = RX / 2;
+  static constexpr int NXL = (RY * XP + NT - 1) / NT;
   using V2 = typename Vec2<T>::type;
   V2 pre[NXL];
   float2 epre[MIXED ? NXL : 1];
@@ -58,7 +58,7 @@ struct XStager {
     const int mc = min(max(m, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
-      int idx = min((int)threadIdx.x + q * MARCH_THREADS, RY * XP - 1);
+      int idx = min((int)threadIdx.x + q * NT, RY * XP - 1);
       int ry = idx / XP, px = idx - ry * XP;
       int gi = i0 - 2 + 2 * px, gj = j0 - 1 + ry;      // gi is odd
       // an odd gi <= nx keeps the 2-element load inside the padded row
@@ -78,7 +78,7 @@ struct XStager {
     const bool vz = mq >= 1 && mq <= L.nz;
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
-      int idx = threadIdx.x + q * MARCH_THREADS;
+      int idx = threadIdx.x + q * NT;
       if (idx < RY * XP) {
         int ry = idx / XP, px = idx - ry * XP;
         int gi = i0 - 2 + 2 * px, gj = j0 - 1 + ry;
@@ -284,23 +284,27 @@ k_refine_start(const double* __restrict__ x, const double* __restrict__ b, doubl
 }
 
 // Residual + restriction for a level that halves every axis.  Fine tile
-// 64 x 16, coarse tile 32 x 8 (one coarse cell per thread); chunks hold an
-// even number of fine planes.
-template <typename T>
+// TX x 16 (TX = 64, or 128 for FP32: the same bytes per row), coarse tile
+// TX/2 x 8, one coarse cell per thread (4 TX threads); chunks hold an even
+// number of fine planes.
+template <int TX>
+constexpr int march_threads() { return 4 * TX; }
+
+template <typename T, int TX>
 __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc, const T* __restrict__ x,
                                                      const T* __restrict__ b, const Lvl<T>& L, const Lvl<T>& Cc,
                                                      const RestrictW<T>& W, const MarchGrid& G) {
-  constexpr int TY = 16;
-  using St = XStager<T, TY, false>;
+  constexpr int TY = 16, TP = TX / 2, NT = march_threads<TX>();
+  using St = XStager<T, TY, false, false, TX, NT>;
   constexpr int RX = St::RX, RY = St::RY;
   constexpr int NP = 2;   // pairs per thread: rows ly and ly + 8
   using V2 = typename Vec2<T>::type;
   __shared__ T sx[4][RY][RX];
-  __shared__ T sres[2][TY][64];
+  __shared__ T sres[2][TY][TX];
 
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
-  const int i0 = 1 + (rxy % G.gx) * 64;
+  const int i0 = 1 + (rxy % G.gx) * TX;
   const int j0 = 1 + (rxy / G.gx) * TY;
   const int k0 = 1 + bz * G.kchunk;                 // odd
   const int k1 = min(k0 + G.kchunk, L.nz + 1);
@@ -311,8 +315,8 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
   T c1a[NP], c2a[NP], c1b[NP], c2b[NP], c3[NP], c4[NP];
 #pragma unroll
   for (int q = 0; q < NP; q++) {
-    ly[q] = t / 32 + 8 * q;
-    lp[q] = t % 32;
+    ly[q] = t / TP + 8 * q;
+    lp[q] = t % TP;
     int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
     act[q] = gj <= L.ny && gi <= L.nx;
     c1a[q] = act[q] ? L.cwx[gi] : T(0);
@@ -322,8 +326,8 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
     c3[q] = act[q] ? L.cwy[gj] : T(0);
     c4[q] = act[q] ? L.cey[gj] : T(0);
   }
-  // coarse cell of this thread: (ic, jc) = (ic0 + t%32, jc0 + t/32)
-  const int lic = t % 32, ljc = t / 32;
+  // coarse cell of this thread: (ic, jc) = (ic0 + t%TP, jc0 + t/TP)
+  const int lic = t % TP, ljc = t / TP;
   const int ic = (i0 + 1) / 2 + lic, jc = (j0 + 1) / 2 + ljc;
   const bool cact = ic <= Cc.nx && jc <= Cc.ny;
   const int iz = 2 * ic - 1, ip = 2 * ic, jz = 2 * jc - 1, jp = 2 * jc;
@@ -365,7 +369,7 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
     for (int q = 0; q < NP; q++) bv[q] = bpre[q];
     st.issue(x, nullptr, L, i0, j0, m + 2);
     issue_b(m + 1);
-    T(*rr)[64] = sres[(m + 1) & 1];   // fine plane 2kc-1 -> slot 0, 2kc -> slot 1
+    T(*rr)[TX] = sres[(m + 1) & 1];   // fine plane 2kc-1 -> slot 0, 2kc -> slot 1
 #pragma unroll
     for (int q = 0; q < NP; q++) {
       T ra = T(0), rb = T(0);
@@ -401,11 +405,11 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(MARCH_THREADS)
+template <typename T, int TX>
+__global__ void __launch_bounds__(march_threads<TX>())
 k_restrict_march(T* __restrict__ bc, const T* __restrict__ x, const T* __restrict__ b, Lvl<T> L, Lvl<T> Cc,
                  RestrictW<T> W, MarchGrid G) {
-  restrict_march_block<T>(blockIdx.x, bc, x, b, L, Cc, W, G);
+  restrict_march_block<T, TX>(blockIdx.x, bc, x, b, L, Cc, W, G);
 }
 
 }  // namespace pepsmg

@@ -192,11 +192,17 @@ struct Ops {
     hipLaunchKernelGGL((k_apply<T>), grid3(d.nx, d.ny, d.nz, 64, 4), dim3(64, 4, 1), 0, s, y, x, d.L());
   }
 
-  // z-march decomposition: 64 x TY tiles, about 2048 blocks, chunk count a
+  // tile width of the restriction and prolongation z-march kernels: 64 cells
+  // in FP64; in FP32 128 (512-byte rows either way) on levels where that
+  // still leaves z-chunks of 8 or more planes (512^3: 20.8 against 21.0 ms
+  // for the mixed solve), 64 below (shorter chunks re-read more halo planes)
+  static constexpr int MTX = sizeof(T) == 8 ? 64 : 128;
+
+  // z-march decomposition: TX x TY tiles, about 2048 blocks, chunk count a
   // multiple of 8 when >= 8; even chunks for the restriction kernel.
-  static MarchGrid march_grid(const DevLevel<T>& d, int TY, bool even) {
+  static MarchGrid march_grid(const DevLevel<T>& d, int TY, bool even, int TX = 64) {
     MarchGrid G;
-    G.gx = (d.nx + 63) / 64;
+    G.gx = (d.nx + TX - 1) / TX;
     G.gy = (d.ny + TY - 1) / TY;
     long nxy = (long)G.gx * G.gy;
     long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + 3) / 4));
@@ -211,9 +217,15 @@ struct Ops {
 
   void residual_restrict(DevLevel<T>& f, DevLevel<T>& c, const T* x, const T* b) {
     if (f.rs[0] == 2 && f.rs[1] == 2 && f.rs[2] == 2) {
-      MarchGrid G = march_grid(f, 16, true);
-      hipLaunchKernelGGL((k_restrict_march<T>), dim3((unsigned)(G.gx * G.gy * G.nch)), dim3(MARCH_THREADS), 0, s,
-                         c.b, x, b, f.L(), c.L(), f.RW(), G);
+      MarchGrid G = march_grid(f, 16, true, MTX);
+      if (MTX == 64 || G.kchunk < 8) {
+        G = march_grid(f, 16, true, 64);
+        hipLaunchKernelGGL((k_restrict_march<T, 64>), dim3((unsigned)(G.gx * G.gy * G.nch)),
+                           dim3(march_threads<64>()), 0, s, c.b, x, b, f.L(), c.L(), f.RW(), G);
+      } else {
+        hipLaunchKernelGGL((k_restrict_march<T, MTX>), dim3((unsigned)(G.gx * G.gy * G.nch)),
+                           dim3(march_threads<MTX>()), 0, s, c.b, x, b, f.L(), c.L(), f.RW(), G);
+      }
       return;
     }
     hipLaunchKernelGGL((k_restrict<T, true>), grid3(c.nx, c.ny, c.nz, 64, 4), dim3(64, 4, 1), 0, s, c.b, x, b,
@@ -238,10 +250,10 @@ struct Ops {
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
   }
 
-  // prolongation march: 64 x 16 fine tiles, chunks of coarse planes
-  static MarchGrid prolong_grid(const DevLevel<T>& f, const DevLevel<T>& c) {
+  // prolongation march: TX x 16 fine tiles, chunks of coarse planes
+  static MarchGrid prolong_grid(const DevLevel<T>& f, const DevLevel<T>& c, int TX = 64) {
     MarchGrid G;
-    G.gx = (f.nx + 63) / 64;
+    G.gx = (f.nx + TX - 1) / TX;
     G.gy = (f.ny + 15) / 16;
     long nxy = (long)G.gx * G.gy;
     long nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (c.nz + 1) / 2));
@@ -253,9 +265,15 @@ struct Ops {
 
   void prolong_add(DevLevel<T>& f, DevLevel<T>& c, T* xf) {
     if (f.ps[0] == 2 && f.ps[1] == 2 && f.ps[2] == 2) {
-      MarchGrid G = prolong_grid(f, c);
-      hipLaunchKernelGGL((k_prolong_march<T>), dim3((unsigned)(G.gx * G.gy * G.nch)), dim3(MARCH_THREADS), 0, s, xf,
-                         c.x, f.L(), c.L(), f.PW(), G);
+      MarchGrid G = prolong_grid(f, c, MTX);
+      if (MTX == 64 || 2 * G.kchunk < 8) {   // chunks count coarse planes
+        G = prolong_grid(f, c, 64);
+        hipLaunchKernelGGL((k_prolong_march<T, 64>), dim3((unsigned)(G.gx * G.gy * G.nch)),
+                           dim3(march_threads<64>()), 0, s, xf, c.x, f.L(), c.L(), f.PW(), G);
+      } else {
+        hipLaunchKernelGGL((k_prolong_march<T, MTX>), dim3((unsigned)(G.gx * G.gy * G.nch)),
+                           dim3(march_threads<MTX>()), 0, s, xf, c.x, f.L(), c.L(), f.PW(), G);
+      }
       return;
     }
     hipLaunchKernelGGL((k_prolong_add<T>), grid3(f.nx, f.ny, f.nz, 64, 4), dim3(64, 4, 1), 0, s, xf, c.x, f.L(),
