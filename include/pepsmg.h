@@ -63,11 +63,12 @@ typedef struct pepsmg_options {
   int32_t max_refine;      /* precision 1: cap on refinement steps (default 30) */
   int32_t fused_smoother;  /* 1 (default): single-pass out-of-place red+black kernel, loads issued one
                               plane ahead; 2: the same with loads two planes ahead; 0: two colour sweeps */
-  int32_t tile_y;          /* fused smoother tile height 8, 16 or 32 cells (0: 16 in FP64, 32 in FP32);
-                              blocks have max(256, tile_x * tile_y / 4) threads; 32-row tiles load one
-                              plane ahead */
+  int32_t tile_y;          /* fused smoother tile height 8, 16 or 32 cells (0: per level, 16 in FP64
+                              and 32 in FP32 on levels of 256 rows or more, 8 below); blocks have
+                              max(256, tile_x * tile_y / 4) threads; 32-row tiles load one plane ahead */
   int32_t verbose;         /* 1: print the reference's [MG] cycle lines */
-  int32_t tile_x;          /* FP32 fused smoother tile width 64 or 128 cells (0: 128); FP64 uses 64 */
+  int32_t tile_x;          /* FP32 fused smoother tile width 64 or 128 cells (0: 128 on levels of 256
+                              rows or more, 64 below); FP64 uses 64 */
   int32_t inner_nu;        /* precision 1: smoothing steps of the FP32 V-cycles (default 2; 0: maxiteration) */
   double inner_omega;      /* precision 1: SOR factor of the FP32 V-cycles' red/black smoothing, in (0, 2)
                               (default 1.35; 0: the caller's omega_sor); the FP64 residual and update,

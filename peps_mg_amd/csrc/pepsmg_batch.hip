@@ -331,13 +331,15 @@ static void rec_smooth_group(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H
 template <typename T>
 static void rec_smooth(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int l, int nu, T omega,
                        bool zero_first) {
-  // tile shape per item: width 64 (FP64 always) or 128 (FP32 default), height 8, 16 or 32
+  // tile shape per item: width 64 (FP64 always) or 128 (FP32), height 8, 16 or 32; unset, the
+  // single-grid solver's default for a level of this size
   std::vector<int> g[2][3];
   for (int i = 0; i < B->nitems; i++) {
     if (l >= B->items[i].s) continue;
     int tx = B->items[i].tile_x ? B->items[i].tile_x : B->opt.tile_x;
     int ty = B->items[i].tile_y ? B->items[i].tile_y : B->opt.tile_y;
-    if (!ty) ty = sizeof(T) == 8 ? 16 : 32;
+    if (!tx) tx = Ops<T>::default_tx(H[i]->lv[l]);
+    if (!ty) ty = Ops<T>::default_ty(H[i]->lv[l]);
     int wx = (sizeof(T) == 4 && tx != 64) ? 1 : 0;
     g[wx][ty == 32 ? 2 : ty == 16 ? 1 : 0].push_back(i);
   }

@@ -714,8 +714,8 @@ int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     n = (double)d.cells();
     if (fused)
-      std::snprintf(name, sizeof name, "k_rb_fused<%s, %d, %d, 0, %d>", tname, op.tx_used(), op.ty_used(),
-                    op.ty_used() == 32 ? 1 : op.prefetch);   // 32-row tiles load one plane ahead
+      std::snprintf(name, sizeof name, "k_rb_fused<%s, %d, %d, 0, %d>", tname, op.tx_used(d), op.ty_used(d),
+                    op.ty_used(d) == 32 ? 1 : op.prefetch);   // 32-row tiles load one plane ahead
     else
       std::snprintf(name, sizeof name, "k_rb_colour<%s, false>", tname);
     return PEPSMG_OK;
