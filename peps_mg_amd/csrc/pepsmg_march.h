@@ -19,6 +19,8 @@
 //                          of (Ap, p) (src/cg_poisson_matrix.f90:206-207)
 //   k_resid_march<T,TY,4>: CG start r = b - A x stored in rout and in xnew (p),
 //                          partial sums of (r, r) (:196-204)
+//   k_resid_march<double,TY,6>: KIND 2, and x_in also stored into the caller's
+//       padded array (the step expected to be the last: no copy-out pass)
 //   k_refine_start:        first iterative-refinement step read straight
 //       from the caller's padded (0:n+1)^3 arrays (unaligned rows): stores x
 //       and b in the solver's layout, r as float and partial r^2 (the mixed
@@ -123,7 +125,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
                                                   float* __restrict__ r32, T* __restrict__ xnew, const Lvl<T>& L,
                                                   const MarchGrid& G, double* __restrict__ partial,
                                                   const Lvl<T>& LI, T* __restrict__ bout) {
-  constexpr bool MIXED = (KIND == 2);
+  constexpr bool MIXED = (KIND == 2 || KIND == 6);
   constexpr bool UNAL = (KIND == 5);
   using St = XStager<T, TY, MIXED, UNAL>;
   constexpr int RX = St::RX, RY = St::RY;
@@ -178,11 +180,12 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
     }
   };
 
-  st.issue(x, e, LI, i0, j0, k0 - 1);
+  const Lvl<T>& LX = UNAL ? LI : L;   // layout of the x input
+  st.issue(x, e, LX, i0, j0, k0 - 1);
   st.stage(sx[(k0 - 1) & 3], L, i0, j0);
-  st.issue(x, e, LI, i0, j0, k0);
+  st.issue(x, e, LX, i0, j0, k0);
   st.stage(sx[k0 & 3], L, i0, j0);
-  st.issue(x, e, LI, i0, j0, k0 + 1);
+  st.issue(x, e, LX, i0, j0, k0 + 1);
   issue_b(k0);
   double acc = 0.0;
   // plane step; the loop below runs it twice per iteration so that the
@@ -198,7 +201,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
     V2 bv[NP];
 #pragma unroll
     for (int q = 0; q < NP; q++) bv[q] = bpre[q];
-    st.issue(x, e, LI, i0, j0, m + 2);
+    st.issue(x, e, LX, i0, j0, m + 2);
     issue_b(m + 1);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
@@ -238,7 +241,12 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
           r32[q0] = (float)ra;
         }
       }
-      if (KIND == 2) {
+      if (KIND == 6) {   // x_in into the caller's padded array (layout LI, base bout)
+        const long u = lidx(LI, gi, gj, m);
+        bout[u] = s0[ly[q] + 1][2 * lp[q] + 2];
+        if (two) bout[u + 1] = s0[ly[q] + 1][2 * lp[q] + 3];
+      }
+      if (KIND == 2 || KIND == 6) {
         if (two) {
           V2 w;
           w.x = s0[ly[q] + 1][2 * lp[q] + 2];
@@ -270,6 +278,16 @@ __global__ void __launch_bounds__(MARCH_THREADS)
 k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __restrict__ b, T* __restrict__ rout,
               float* __restrict__ r32, T* __restrict__ xnew, Lvl<T> L, MarchGrid G, double* __restrict__ partial) {
   resid_march_block<T, TY, KIND>(blockIdx.x, x, e, b, rout, r32, xnew, L, G, partial, L, (T*)nullptr);
+}
+
+// Refinement step (KIND 2) that also stores x_in into the caller's padded
+// array usr (layout U: pitches nx+2 and (nx+2)(ny+2), base at cell (1,1,1)).
+template <int TY>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_refine_last(const double* __restrict__ x, const float* __restrict__ e, const double* __restrict__ b,
+              float* __restrict__ r32, double* __restrict__ xnew, Lvl<double> L, Lvl<double> U, MarchGrid G,
+              double* __restrict__ partial, double* __restrict__ usr) {
+  resid_march_block<double, TY, 6>(blockIdx.x, x, e, b, (double*)nullptr, r32, xnew, L, G, partial, U, usr);
 }
 
 // First refinement step from the caller's arrays: U is the layout of x and b

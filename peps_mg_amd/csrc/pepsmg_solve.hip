@@ -401,9 +401,11 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
 // b - A x, the update x += e and the coarsest solve stay in FP64.
 // ux, ub: the caller's padded x and b (device), read directly by the first
 // refinement step; nullptr: x and b are already in the solver's layout.
+// The step predicted to be the last also stores its x into ux, and
+// c->sol_out_done records whether the solve ended on such a step.
 static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vector<double>& hist,
                        std::vector<int>& cit, double* res0_out, int* ncyc_out, int* nref_out, double* rho_out,
-                       const double* ux, const double* ub) {
+                       double* ux, const double* ub) {
   Hier<double>& h = c->h64;
   Hier<float>& g = c->h32;
   Ops<float> op{c->stream};
@@ -421,25 +423,33 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   MarchGrid G = Ops<double>::march_grid(f, RESID_TY, false);
   int nb = G.gx * G.gy * G.nch;
   // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x, x2
-  auto refine_update = [&]() {
-    hipLaunchKernelGGL((k_resid_march<double, RESID_TY, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
-                       f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
+  // the caller's padded layout (pitches nx+2, (nx+2)(ny+2); offset of cell (1,1,1))
+  Lvl<double> U = f.L();
+  U.px = f.nx + 2;
+  U.pxy = (long)(f.nx + 2) * (f.ny + 2);
+  const long uo = 1 + U.px + U.pxy;
+  c->sol_out_done = false;
+  // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x,
+  // x2.  last: x2 also goes into the caller's array
+  auto refine_update = [&](bool last) {
+    if (last)
+      hipLaunchKernelGGL((k_refine_last<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
+                         f32.x, f.b, f32.b, f.x2, f.L(), U, G, c->d_partial, ux + uo);
+    else
+      hipLaunchKernelGGL((k_resid_march<double, RESID_TY, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0,
+                         c->stream, f.x, f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
     std::swap(f.x, f.x2);
   };
   // r32 = b - A x and res0, with e = 0
   if (ux) {   // x and b come from the caller's padded arrays and are stored in the solver's layout
-    Lvl<double> U = f.L();
-    U.px = f.nx + 2;
-    U.pxy = (long)(f.nx + 2) * (f.ny + 2);
-    const long o = 1 + U.px + U.pxy;   // cell (1,1,1)
-    hipLaunchKernelGGL((k_refine_start<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, ux + o,
-                       ub + o, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
+    hipLaunchKernelGGL((k_refine_start<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, ux + uo,
+                       ub + uo, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
     std::swap(f.x, f.x2);
   } else {
     hipMemsetAsync(f32.x, 0, f32.ncell * sizeof(float), c->stream);
-    refine_update();
+    refine_update(false);
   }
   // SOR factor of the FP32 inner cycles (an inner-solver setting: the FP64
   // residual and update, which decide convergence, do not use it)
@@ -491,9 +501,11 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
       smooth_level(c, op, g, cg, 0, nui, om, false, true);
       ncyc++;
     }
-    refine_update();
     // predicted to stop here: the previous per-cycle factor over this step's cycles, factor-2 margin
     const bool stop_likely = fcyc > 0.0 && rho_prev * std::pow(fcyc, nin) < 2.0 * tol;
+    // (never into an array the entry point will reject: its ghost flag has arrived with the first norm)
+    const bool last = stop_likely && ux != nullptr && *c->ghost_flag_host() == 0;
+    refine_update(last);
     rc = read_norm(!stop_likely && s < c->opt.max_refine && ncyc < c->ncycle);
     if (rc) return rc;
     const double r2 = c->h_scalars[0];
@@ -501,7 +513,10 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     hist.push_back(rho);
     nref = s;
     if (c->opt.verbose) print_cycle(s, r2, res0);
-    if (rho < tol) break;
+    if (rho < tol) {
+      c->sol_out_done = last;
+      break;
+    }
     if (rho > 0.0 && rho_prev > 0.0) fcyc = std::pow(rho / rho_prev, 1.0 / nin);
   }
   if (ncyc > 0) {
@@ -559,7 +574,7 @@ static int check_ghosts_host(const double* a, int nx, int ny, int nz, const char
   return PEPSMG_OK;
 }
 
-static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_stats* st, const double* ux = nullptr,
+static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_stats* st, double* ux = nullptr,
                      const double* ub = nullptr) {
   std::vector<double> hist;
   std::vector<int> cit;
@@ -627,7 +642,7 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
   }
   // the flag is read back with the solve's first norm (the solve never
   // reads ghost cells and writes nothing of the caller's before the check)
-  int* h_flag = reinterpret_cast<int*>(c->h_scalars + 7);
+  int* h_flag = c->ghost_flag_host();
   *h_flag = 0;
   hipError_t e = hipMemcpyAsync(h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e != hipSuccess) rc = hip_fail(e, "pack");
@@ -639,7 +654,7 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
     else if (flag) rc = fail(PEPSMG_ERR_GHOST, "sol has non-zero ghost cells (homogeneous Dirichlet required)");
   }
   if (!rc) {
-    unpack(c->stream, d_sol, f.x, f, 0);
+    if (!(direct && c->sol_out_done)) unpack(c->stream, d_sol, f.x, f, 0);   // else the last step stored it
     if (d_rsd) {
       Ops<double> op{c->stream};
       op.residual(f, f.r, f.x, f.b);                                   // :1970
