@@ -287,3 +287,29 @@ def test_device_solve_orders_after_default_stream_work(precision):
         assert torch.equal(x, ref)
         del y
     mg.destroy()
+
+
+@pytest.mark.parametrize("dims,nlevel", [((256, 256, 128), 4), ((512, 256, 64), 6)])
+def test_fp64_decompositions_agree_large(dims, nlevel):
+    """On grids where the smoother's one-wave decomposition and the per-level
+    tile choice apply, the FP64 solve gives the same solution bit for bit as
+    fixed tiles and as the two-sweep colour smoother (every tiling computes
+    the same values; only the block layout differs)."""
+    import torch
+    nx, ny, nz = dims
+    g = O.Grid(nx, ny, nz, l=(nx / 256, ny / 256, nz / 256))   # cubic cells
+    sdm = sdm_from(g)
+    b = torch.from_numpy(-O.uniform_field(nx, ny, nz, 5)).cuda()
+    out = []
+    for opts in (dict(), dict(tile_y=8), dict(tile_y=32), dict(fused_smoother=0)):
+        mg = Multigrid(sdm, nlevel, 60, **opts)
+        x = torch.zeros_like(b)
+        r = mg.solve_vcycle_device(x.data_ptr(), 0, b.data_ptr(), 2, 1e-10, 1.0)
+        torch.cuda.synchronize()
+        assert r.converged
+        out.append((x, r))
+        mg.destroy()
+    for x, r in out[1:]:
+        assert r.cycles == out[0][1].cycles
+        np.testing.assert_allclose(r.history, out[0][1].history, rtol=1e-12, atol=0)
+        assert torch.equal(x, out[0][0])
