@@ -63,8 +63,8 @@ typedef struct pepsmg_options {
   int32_t max_refine;      /* precision 1: cap on refinement steps (default 30) */
   int32_t fused_smoother;  /* 1 (default): single-pass out-of-place red+black kernel, loads issued one
                               plane ahead; 2: the same with loads two planes ahead; 0: two colour sweeps */
-  int32_t tile_y;          /* fused smoother tile height 8, 16 or 32 cells (0: per level, 16 in FP64
-                              and 32 in FP32 on levels of 256 rows or more, 8 below); blocks have
+  int32_t tile_y;          /* fused smoother tile height 8, 16 or 32 cells (0: per level, 32 on levels
+                              of 256 rows or more, 8 below); blocks have
                               max(256, tile_x * tile_y / 4) threads; 32-row tiles load one plane ahead */
   int32_t verbose;         /* 1: print the reference's [MG] cycle lines */
   int32_t tile_x;          /* FP32 fused smoother tile width 64 or 128 cells (0: 128 on levels of 256

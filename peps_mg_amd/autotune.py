@@ -25,7 +25,7 @@ import time
 
 from . import _lib
 
-TILES_FP64 = [(64, 16, 1), (64, 8, 1), (64, 16, 2), (64, 32, 1)]
+TILES_FP64 = [(64, 32, 1), (64, 16, 1), (64, 8, 1), (64, 16, 2)]
 TILES_FP32 = [(128, 32, 1), (128, 16, 1), (128, 8, 1), (64, 8, 1), (64, 16, 1), (128, 16, 2), (64, 32, 1)]
 
 

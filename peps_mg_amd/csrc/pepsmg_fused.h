@@ -25,9 +25,8 @@
 // double-buffered for 2).  Per-thread data that depends on the plane only
 // through its parity (which cell of a pair is red, coefficients, partial
 // diagonal sums, LDS and b offsets) is precomputed for both parities.
-// Blocks are numbered with the z-chunk fastest, so with 8 (or a multiple of
-// 8) chunks each XCD (blocks are dealt round-robin over the 8 XCDs) owns
-// whole z-slabs and the tile rings are re-read from its own L2.
+// Workgroups map to (tile, chunk) with the chunk fastest or, for tiles of
+// 2048 cells or more, XCD-aware (fused_xcd_bid).
 //
 // MODE 0: plain.  MODE 1: x_in = 0 (the zeroed coarse x of
 // src/multigrid.f90:1937; nothing is read).  MODE 2: plain, and each block
@@ -46,7 +45,7 @@ namespace pepsmg {
 constexpr int FUSED_THREADS = 256;
 // threads of a TX x TY tile: two black pairs per thread (at least 256
 // threads), so taller or wider tiles keep the per-thread registers of the
-// 64 x 16 (FP64) and 128 x 8 (FP32) ones
+// 64 x 16 and 128 x 8 ones
 constexpr int fused_threads(int tx, int ty) { return tx * ty / 4 > FUSED_THREADS ? tx * ty / 4 : FUSED_THREADS; }
 
 template <typename T>
@@ -69,6 +68,7 @@ __device__ __forceinline__ typename Vec2<T>::type ld_stream_v2(const T* p) {
 
 struct FusedGrid {
   int gx, gy, nch, kchunk;
+  int order = 0;   // workgroup order (fused_xcd_bid)
 };
 
 // omega * (b - temp) / c0 + (1 - omega) * xold, the reference's SOR update
@@ -410,11 +410,27 @@ __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, cons
   }
 }
 
+// Block number of workgroup w in a grid of nb blocks.  order 0: w itself
+// (z-chunk fastest).  order 1, XCD-aware: the hardware deals workgroups
+// round-robin over the 8 XCDs, so XCD w % 8 gets the w / 8-th block of a
+// contiguous eighth of the order x-tiles fastest, then y-tiles, then
+// z-chunks: each XCD owns full-width bands of tiles of one slab and re-reads
+// the column rings beside a tile from its own L2.  The block number keeps
+// the chunk fastest either way (the partial-sum slot of a tile and chunk,
+// and so the summation order of the norm, do not depend on the order).
+__device__ __forceinline__ int fused_xcd_bid(int w, int nb, const FusedGrid& G) {
+  if (G.order == 0 || (nb & 7)) return w;
+  const int l = (w & 7) * (nb >> 3) + (w >> 3);
+  const int nxy = G.gx * G.gy;
+  const int bz = l / nxy;
+  return (l - bz * nxy) * G.nch + bz;
+}
+
 template <typename T, int TX, int TY, int MODE, int PF>
 __global__ void __launch_bounds__(fused_threads(TX, TY))
 k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G,
            double* __restrict__ partial) {
-  rb_fused_block<T, TX, TY, MODE, PF>(blockIdx.x, xo, xi, b, L, omega, G, partial);
+  rb_fused_block<T, TX, TY, MODE, PF>(fused_xcd_bid(blockIdx.x, gridDim.x, G), xo, xi, b, L, omega, G, partial);
 }
 
 }  // namespace pepsmg
