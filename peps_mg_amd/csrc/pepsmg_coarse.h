@@ -186,6 +186,8 @@ k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> C
 // mixed-precision inner cycles).  Operations run in the reference order and
 // the coarsest norm of up to 64 cells is summed sequentially, so FP64
 // results are bit-identical to the per-level kernels and to the reference.
+// The 1-D coefficient and transfer-weight arrays of every level are staged
+// into LDS at the start, so no phase waits on a global load.
 constexpr int PC_MAXLEV = 8;
 constexpr int PC_THREADS = 1024;
 
@@ -201,31 +203,240 @@ struct PCArgs {
   RestrictW<T> rw[PC_MAXLEV];     // level q -> q+1
   ProlongW<T> pw[PC_MAXLEV];      // level q <- q+1
   int xoff[PC_MAXLEV], boff[PC_MAXLEV];   // LDS offsets (elements of T)
+  int coff[PC_MAXLEV];            // staged coefficients and weights of levels 0..nlev-2 (T)
+  int roff;                       // residual of the level being restricted (T, level 0's size)
   int xcoff, bcoff, xctoff;       // coarsest: FP64 x, b (in doubles), T copy of x (in T)
+  int ccoff;                      // staged coarsest coefficients (doubles)
   Lvl<T> gl;                      // global geometry of level s
   T* gxs;                         // global x_s (output)
   const T* gbs;                   // global b_s (input)
   int* iters_slot;
 };
 
-__device__ __forceinline__ int pc_i(int c, int nx) { return c % nx + 1; }
+
+// Staged arrays of one level in LDS (axis lengths ex = nx+2, ey, ez):
+// cwx cex | cwy cey | cwz cez, then for levels with a coarser one the
+// restriction weights wx | wy | wz and the prolongation weights
+// px nx | py ny | pz nz.  pc_stage_elems gives the length of the block.
+__host__ __device__ inline int pc_stage_elems(int nx, int ny, int nz, bool transfer) {
+  return (transfer ? 5 : 2) * (nx + ny + nz + 6);
+}
+
+// copies level G's arrays (and transfer weights when R, P are given) to s.
+// Every load of a thread is issued before its first LDS store (clamped
+// indices, no branch next to a load), so staging costs one load latency.
+template <typename T>
+__device__ void pc_stage(T* s, const Lvl<T>& G, const RestrictW<T>* R, const ProlongW<T>* P) {
+  const int ex = G.nx + 2, ey = G.ny + 2, ez = G.nz + 2;
+  const int em = max(ex, max(ey, ez));
+  for (int c = threadIdx.x; c < em; c += blockDim.x) {
+    const int cx = min(c, ex - 1), cy = min(c, ey - 1), cz = min(c, ez - 1);
+    T v[15];
+    v[0] = G.cwx[cx];
+    v[1] = G.cex[cx];
+    v[2] = G.cwy[cy];
+    v[3] = G.cey[cy];
+    v[4] = G.cwz[cz];
+    v[5] = G.cez[cz];
+    if (R) {
+      v[6] = R->wx[cx];
+      v[7] = R->wy[cy];
+      v[8] = R->wz[cz];
+      v[9] = P->px_[cx];
+      v[10] = P->nx_[cx];
+      v[11] = P->py_[cy];
+      v[12] = P->ny_[cy];
+      v[13] = P->pz_[cz];
+      v[14] = P->nz_[cz];
+    }
+    if (c < ex) {
+      s[c] = v[0];
+      s[ex + c] = v[1];
+    }
+    if (c < ey) {
+      s[2 * ex + c] = v[2];
+      s[2 * ex + ey + c] = v[3];
+    }
+    if (c < ez) {
+      s[2 * (ex + ey) + c] = v[4];
+      s[2 * (ex + ey) + ez + c] = v[5];
+    }
+    if (R) {
+      T* w = s + 2 * (ex + ey + ez);
+      T* p = w + (ex + ey + ez);
+      if (c < ex) {
+        w[c] = v[6];
+        p[c] = v[9];
+        p[ex + c] = v[10];
+      }
+      if (c < ey) {
+        w[ex + c] = v[7];
+        p[2 * ex + c] = v[11];
+        p[2 * ex + ey + c] = v[12];
+      }
+      if (c < ez) {
+        w[ex + ey + c] = v[8];
+        p[2 * (ex + ey) + c] = v[13];
+        p[2 * (ex + ey) + ez + c] = v[14];
+      }
+    }
+  }
+}
+
+// level geometry G with its coefficient pointers moved to the staged copy s
+template <typename T>
+__device__ __forceinline__ Lvl<T> pc_lvl(const Lvl<T>& G, const T* s) {
+  const int ex = G.nx + 2, ey = G.ny + 2, ez = G.nz + 2;
+  Lvl<T> L = G;
+  L.cwx = s;
+  L.cex = s + ex;
+  L.cwy = s + 2 * ex;
+  L.cey = s + 2 * ex + ey;
+  L.cwz = s + 2 * (ex + ey);
+  L.cez = s + 2 * (ex + ey) + ez;
+  return L;
+}
+template <typename T>
+__device__ __forceinline__ RestrictW<T> pc_rw(const RestrictW<T>& W, const Lvl<T>& G, const T* s) {
+  const int ex = G.nx + 2, ey = G.ny + 2, ez = G.nz + 2;
+  RestrictW<T> R = W;
+  const T* w = s + 2 * (ex + ey + ez);
+  R.wx = w;
+  R.wy = w + ex;
+  R.wz = w + ex + ey;
+  return R;
+}
+template <typename T>
+__device__ __forceinline__ ProlongW<T> pc_pw(const ProlongW<T>& W, const Lvl<T>& G, const T* s) {
+  const int ex = G.nx + 2, ey = G.ny + 2, ez = G.nz + 2;
+  ProlongW<T> P = W;
+  const T* p = s + 3 * (ex + ey + ez);
+  P.px_ = p;
+  P.nx_ = p + ex;
+  P.py_ = p + 2 * ex;
+  P.ny_ = p + 2 * ex + ey;
+  P.pz_ = p + 2 * (ex + ey);
+  P.nz_ = p + 2 * (ex + ey) + ez;
+  return P;
+}
+
+// value of lane l of v (l a compile-time lane after unrolling)
+__device__ __forceinline__ double pc_readlane(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffu), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Coarsest rbgs_solver on up to 64 cells (src/rbgs_poisson_matrix.f90:47-179)
+// by one wavefront, cell c on lane c: coefficients, c0 and b stay in
+// registers; x lives in LDS (one wavefront orders its own LDS traffic, so a
+// wavefront fence replaces the workgroup barriers).  The norm sums the lanes'
+// r^2 in cell order (lanes without a cell add +0.0, which leaves the
+// non-negative sum unchanged): the sequential sum of the reference, bit for
+// bit.  Returns the iteration count (the reference's loop variable).
+static __device__ __forceinline__ int pc_coarsest_wave(double* __restrict__ x, const double* __restrict__ b,
+                                             const Lvl<double>& L, double omega, double tol, int maxit) {
+  const int n = L.nx * L.ny * L.nz;
+  const int lane = threadIdx.x;
+  const bool own = lane < n;
+  const int c = own ? lane : 0;
+  const int i = c % L.nx + 1, j = (c / L.nx) % L.ny + 1, k = c / (L.nx * L.ny) + 1;
+  const int col = (i + j + k) & 1;     // 1: first sweep (i+j+k odd), 0: second
+  const double c1 = L.cwx[i], c2 = L.cex[i], c3 = L.cwy[j], c4 = L.cey[j], c5 = L.cwz[k], c6 = L.cez[k];
+  const double c0 = diag(c1, c2, c3, c4, c5, c6);
+  const double bv = b[c];
+  const int sy = L.nx, sz = L.nx * L.ny;
+  // neighbour cells; -1 outside the level (value 0, the homogeneous ghost)
+  const int nw = i > 1 ? c - 1 : -1, ne = i < L.nx ? c + 1 : -1;
+  const int ns = j > 1 ? c - sy : -1, nn = j < L.ny ? c + sy : -1;
+  const int nb = k > 1 ? c - sz : -1, nt = k < L.nz ? c + sz : -1;
+  auto X = [&](int o) -> double { return o >= 0 ? x[o] : 0.0; };
+  auto wsync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+  auto norm = [&]() -> double {
+    double r = 0.0;
+    if (own) {
+      const double y = c0 * x[c] + c1 * X(nw) + c2 * X(ne) + c3 * X(ns) + c4 * X(nn) + c5 * X(nb) + c6 * X(nt);
+      r = bv - y;
+    }
+    const double r2 = r * r;
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < 64; l++) s = s + pc_readlane(r2, l);
+    return s;
+  };
+  auto sweep = [&](int colour) {
+    if (own && col == colour) {
+      const double temp = c1 * X(nw) + c2 * X(ne) + c3 * X(ns) + c4 * X(nn) + c5 * X(nb) + c6 * X(nt);
+      x[c] = omega * (bv - temp) / c0 + (1.0 - omega) * x[c];
+    }
+    wsync();
+  };
+  const double r0 = norm();
+  int it;
+  for (it = 0; it < maxit; it++) {
+    sweep(1);
+    sweep(0);
+    const double rn = norm();
+    if (sqrt(rn / r0) <= tol) break;
+  }
+  return it;
+}
+
+// floor(n / d) for 0 <= n, d < 2^16 (levels held in LDS are far smaller):
+// one 64-bit multiply by m = floor((2^32 - 1) / d) + 1 instead of an integer
+// division (the error n * (m - 2^32 / d) / 2^32 < n / 2^32 stays below 1/d)
+__device__ __forceinline__ unsigned long long pc_magic(int d) { return 0xFFFFFFFFull / (unsigned)d + 1ull; }
+__device__ __forceinline__ int pc_div(int n, unsigned long long m) {
+  return (int)(((unsigned long long)(unsigned)n * m) >> 32);
+}
+
+// Cells of a compact LDS level (px = nx, pxy = nx ny): cell q = (i-1) +
+// nx (j-1) + nx ny (k-1); neighbours outside the level read as 0, the
+// homogeneous ghost (ldz).  Values and operation order are those of
+// sor_at / apply_at.
+template <typename T>
+struct PCNb {
+  T w, e, s, n, b, t;
+};
+template <typename T>
+__device__ __forceinline__ PCNb<T> pc_nb(const T* x, const Lvl<T>& L, int i, int j, int k, int q) {
+  const int sy = L.nx, sz = L.nx * L.ny;
+  PCNb<T> v;
+  v.w = i > 1 ? x[q - 1] : T(0);
+  v.e = i < L.nx ? x[q + 1] : T(0);
+  v.s = j > 1 ? x[q - sy] : T(0);
+  v.n = j < L.ny ? x[q + sy] : T(0);
+  v.b = k > 1 ? x[q - sz] : T(0);
+  v.t = k < L.nz ? x[q + sz] : T(0);
+  return v;
+}
 
 template <typename T>
 __device__ void pc_colour(T* x, const T* b, const Lvl<T>& L, T omega, int colour, bool zero) {
-  int hx = (L.nx + 1) / 2;
-  int n = hx * L.ny * L.nz;
+  const int hx = (L.nx + 1) / 2;
+  const int n = hx * L.ny * L.nz;
+  const unsigned long long mh = pc_magic(hx), my = pc_magic(L.ny);
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    int tt = c % hx, j = (c / hx) % L.ny + 1, k = c / (hx * L.ny) + 1;
-    int i = 1 + ((j + k + colour) & 1) + 2 * tt;
+    const int row = pc_div(c, mh);                 // (j-1) + ny (k-1)
+    const int tt = c - row * hx, kk = pc_div(row, my);
+    const int j = row - kk * L.ny + 1, k = kk + 1;
+    const int i = 1 + ((j + k + colour) & 1) + 2 * tt;
     if (i <= L.nx) {
-      long q = lidx(L, i, j, k);
+      const int q = (i - 1) + L.nx * row;
+      const T c1 = L.cwx[i], c2 = L.cex[i], c3 = L.cwy[j], c4 = L.cey[j], c5 = L.cwz[k], c6 = L.cez[k];
       if (zero) {
         T z = T(0);
-        T c1 = L.cwx[i], c2 = L.cex[i], c3 = L.cwy[j], c4 = L.cey[j], c5 = L.cwz[k], c6 = L.cez[k];
         T temp = c1 * z + c2 * z + c3 * z + c4 * z + c5 * z + c6 * z;
         x[q] = omega * (b[q] - temp) / diag(c1, c2, c3, c4, c5, c6) + (T(1) - omega) * z;
       } else {
-        x[q] = sor_at(x, b, L, i, j, k, omega, x[q]);
+        const T c0 = diag(c1, c2, c3, c4, c5, c6);
+        const PCNb<T> v = pc_nb(x, L, i, j, k, q);
+        const T temp = c1 * v.w + c2 * v.e + c3 * v.s + c4 * v.n + c5 * v.b + c6 * v.t;
+        x[q] = omega * (b[q] - temp) / c0 + (T(1) - omega) * x[q];
       }
     }
   }
@@ -248,12 +459,29 @@ __device__ void pc_smooth(T* x, const T* b, const Lvl<T>& L, T omega, int nu, bo
   }
 }
 
+// residual of every fine cell into r (compact, one per thread in turn), then
+// the restriction from r: the same values and sums as the per-level kernels
 template <typename T, typename TC>
-__device__ void pc_restrict(TC* bc, const T* x, const T* b, const Lvl<T>& F, int cnx, int cny, int cnz,
+__device__ void pc_restrict(TC* bc, T* r, const T* x, const T* b, const Lvl<T>& F, int cnx, int cny, int cnz,
                             const RestrictW<T>& W) {
-  int n = cnx * cny * cnz;
+  const int nf = F.nx * F.ny * F.nz;
+  const unsigned long long mx = pc_magic(F.nx), my = pc_magic(F.ny);
+  for (int c = threadIdx.x; c < nf; c += blockDim.x) {
+    const int row = pc_div(c, mx), kk = pc_div(row, my);
+    const int i = c - row * F.nx + 1, j = row - kk * F.ny + 1, k = kk + 1;
+    const T c1 = F.cwx[i], c2 = F.cex[i], c3 = F.cwy[j], c4 = F.cey[j], c5 = F.cwz[k], c6 = F.cez[k];
+    const T c0 = diag(c1, c2, c3, c4, c5, c6);
+    const PCNb<T> v = pc_nb(x, F, i, j, k, c);
+    const T y = c0 * x[c] + c1 * v.w + c2 * v.e + c3 * v.s + c4 * v.n + c5 * v.b + c6 * v.t;
+    r[c] = b[c] - y;
+  }
+  __syncthreads();
+  const int n = cnx * cny * cnz;
+  const unsigned long long mcx = pc_magic(cnx), mcy = pc_magic(cny);
+  const int sy = F.nx, sz = F.nx * F.ny;
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    int ic = c % cnx + 1, jc = (c / cnx) % cny + 1, kc = c / (cnx * cny) + 1;
+    const int row = pc_div(c, mcx), kk = pc_div(row, mcy);
+    const int ic = c - row * cnx + 1, jc = row - kk * cny + 1, kc = kk + 1;
     int iz, ip, jz, jp, kz, kp;
     T ax1, ax2, ay1, ay2, az1, az2;
     restrict_axis(W.wx, W.sx, ic, iz, ip, ax1, ax2);
@@ -262,22 +490,36 @@ __device__ void pc_restrict(TC* bc, const T* x, const T* b, const Lvl<T>& F, int
     T v111 = ax1 * ay1 * az1, v211 = ax2 * ay1 * az1, v121 = ax1 * ay2 * az1, v221 = ax2 * ay2 * az1;
     T v112 = ax1 * ay1 * az2, v212 = ax2 * ay1 * az2, v122 = ax1 * ay2 * az2, v222 = ax2 * ay2 * az2;
     T vol_c = v111 + v211 + v121 + v221 + v112 + v212 + v122 + v222;
-    T s = v111 * resid_at(x, b, F, ip, jp, kp) + v211 * resid_at(x, b, F, iz, jp, kp) +
-          v121 * resid_at(x, b, F, ip, jz, kp) + v221 * resid_at(x, b, F, iz, jz, kp) +
-          v112 * resid_at(x, b, F, ip, jp, kz) + v212 * resid_at(x, b, F, iz, jp, kz) +
-          v122 * resid_at(x, b, F, ip, jz, kz) + v222 * resid_at(x, b, F, iz, jz, kz);
-    bc[(ic - 1) + cnx * ((jc - 1) + cny * (kc - 1))] = (TC)(s / vol_c);
+    auto R = [&](int a, int bb, int cc) { return r[(a - 1) + sy * (bb - 1) + sz * (cc - 1)]; };
+    T s = v111 * R(ip, jp, kp) + v211 * R(iz, jp, kp) + v121 * R(ip, jz, kp) + v221 * R(iz, jz, kp) +
+          v112 * R(ip, jp, kz) + v212 * R(iz, jp, kz) + v122 * R(ip, jz, kz) + v222 * R(iz, jz, kz);
+    bc[c] = (TC)(s / vol_c);
   }
   __syncthreads();
 }
 
+// x_f += P x_c (prolong_at's weights, terms and order) on a compact level
 template <typename T>
 __device__ void pc_prolong_add(T* xf, const T* xc, const Lvl<T>& F, const Lvl<T>& Cc, const ProlongW<T>& W) {
-  int n = F.nx * F.ny * F.nz;
+  const int n = F.nx * F.ny * F.nz;
+  const unsigned long long mx = pc_magic(F.nx), my = pc_magic(F.ny);
+  auto V = [&](int a, int bb, int cc) -> T {
+    return (a >= 1 && a <= Cc.nx && bb >= 1 && bb <= Cc.ny && cc >= 1 && cc <= Cc.nz)
+               ? xc[(a - 1) + Cc.nx * ((bb - 1) + Cc.ny * (cc - 1))]
+               : T(0);
+  };
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    int i = c % F.nx + 1, j = (c / F.nx) % F.ny + 1, k = c / (F.nx * F.ny) + 1;
-    long q = lidx(F, i, j, k);
-    xf[q] = xf[q] + prolong_at(xc, Cc, W, i, j, k);
+    const int row = pc_div(c, mx), kk = pc_div(row, my);
+    const int i = c - row * F.nx + 1, j = row - kk * F.ny + 1, k = kk + 1;
+    int ci, ni, cj, nj, ck, nk;
+    T wsx, wnx, wsy, wny, wsz, wnz;
+    prolong_axis(W.px_, W.nx_, W.sx, i, ci, ni, wsx, wnx);
+    prolong_axis(W.py_, W.ny_, W.sy, j, cj, nj, wsy, wny);
+    prolong_axis(W.pz_, W.nz_, W.sz, k, ck, nk, wsz, wnz);
+    T num = wsx * wsy * wsz * V(ci, cj, ck) + wnx * wsy * wsz * V(ni, cj, ck) + wsx * wny * wsz * V(ci, nj, ck) +
+            wsx * wsy * wnz * V(ci, cj, nk) + wnx * wny * wsz * V(ni, nj, ck) + wnx * wsy * wnz * V(ni, cj, nk) +
+            wsx * wny * wnz * V(ci, nj, nk) + wnx * wny * wnz * V(ni, nj, nk);
+    xf[c] = xf[c] + num / ((wsx + wnx) * (wsy + wny) * (wsz + wnz));
   }
   __syncthreads();
 }
@@ -290,34 +532,51 @@ __device__ __forceinline__ void coarse_vcycle_block(const PCArgs<T>& A) {
   __shared__ double sh[PC_THREADS / 64];
   __shared__ double seq[64];
   const int top = A.nlev - 1;                     // index of the coarsest level
-  // b_s from global
+  // coefficients and transfer weights into LDS, b_s from global
+  for (int q = 0; q < top; q++) pc_stage(sm + A.coff[q], A.lv[q], &A.rw[q], &A.pw[q]);
+  pc_stage(smd + A.ccoff, A.lc, (const RestrictW<double>*)nullptr, (const ProlongW<double>*)nullptr);
   {
+    // four loads in flight per thread (clamped cells) before the LDS stores
     const Lvl<T>& L = A.lv[0];
-    int n = L.nx * L.ny * L.nz;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-      int i = c % L.nx + 1, j = (c / L.nx) % L.ny + 1, k = c / (L.nx * L.ny) + 1;
-      T v = A.gbs[lidx(A.gl, i, j, k)];
-      if (top == 0) smd[A.bcoff + c] = (double)v;
-      else sm[A.boff[0] + c] = v;
+    const int n = L.nx * L.ny * L.nz;
+    const int bd = blockDim.x;
+    for (int c0 = threadIdx.x; c0 < n; c0 += 4 * bd) {
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int c = min(c0 + u * bd, n - 1);
+        const int i = c % L.nx + 1, j = (c / L.nx) % L.ny + 1, k = c / (L.nx * L.ny) + 1;
+        v[u] = A.gbs[lidx(A.gl, i, j, k)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int c = c0 + u * bd;
+        if (c < n) {
+          if (top == 0) smd[A.bcoff + c] = (double)v[u];
+          else sm[A.boff[0] + c] = v[u];
+        }
+      }
     }
     __syncthreads();
   }
   // down-leg: levels 0 .. top-1 (local numbering)
   for (int q = 0; q < top; q++) {
-    const Lvl<T>& L = A.lv[q];
+    const T* st = sm + A.coff[q];
+    const Lvl<T> L = pc_lvl(A.lv[q], st);
+    const RestrictW<T> W = pc_rw(A.rw[q], A.lv[q], st);
     T* x = sm + A.xoff[q];
     const T* b = sm + A.boff[q];
     pc_smooth(x, b, L, A.omega, A.nu, true);
     if (q + 1 < top) {
       const Lvl<T>& C = A.lv[q + 1];
-      pc_restrict<T, T>(sm + A.boff[q + 1], x, b, L, C.nx, C.ny, C.nz, A.rw[q]);
+      pc_restrict<T, T>(sm + A.boff[q + 1], sm + A.roff, x, b, L, C.nx, C.ny, C.nz, W);
     } else {
-      pc_restrict<T, double>(smd + A.bcoff, x, b, L, A.lc.nx, A.lc.ny, A.lc.nz, A.rw[q]);
+      pc_restrict<T, double>(smd + A.bcoff, sm + A.roff, x, b, L, A.lc.nx, A.lc.ny, A.lc.nz, W);
     }
   }
   // coarsest solve in FP64
   {
-    const Lvl<double>& L = A.lc;
+    const Lvl<double> L = pc_lvl(A.lc, (const double*)(smd + A.ccoff));
     double* x = smd + A.xcoff;
     const double* b = smd + A.bcoff;
     int n = L.nx * L.ny * L.nz;
@@ -329,39 +588,9 @@ __device__ __forceinline__ void coarse_vcycle_block(const PCArgs<T>& A) {
       }
       __syncthreads();
     } else if (n <= 64) {
-      // one wavefront, one cell per lane; LDS traffic of a single wave is
-      // ordered, so a wavefront fence replaces the workgroup barriers
       if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        const bool own = lane < n;
-        const int i = lane % L.nx + 1, j = (lane / L.nx) % L.ny + 1, k = lane / (L.nx * L.ny) + 1;
-        const int col = (i + j + k) & 1;     // 1: first sweep (i+j+k odd), 0: second
-        auto wsync = []() {
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-        };
-        auto norm = [&]() -> double {
-          double r = own ? resid_at(x, b, L, i, j, k) : 0.0;
-          seq[lane] = r * r;
-          wsync();
-          double s = 0.0;
-          if (lane == 0)
-            for (int c = 0; c < n; c++) s = s + seq[c];
-          s = __shfl(s, 0, 64);
-          wsync();
-          return s;
-        };
-        double r0 = norm();
-        int it;
-        for (it = 0; it < A.maxit; it++) {
-          if (own && col == 1) x[lidx(L, i, j, k)] = sor_at(x, b, L, i, j, k, (double)A.omega, x[lidx(L, i, j, k)]);
-          wsync();
-          if (own && col == 0) x[lidx(L, i, j, k)] = sor_at(x, b, L, i, j, k, (double)A.omega, x[lidx(L, i, j, k)]);
-          wsync();
-          double rn = norm();
-          if (sqrt(rn / r0) <= A.tol) break;
-        }
-        if (lane == 0 && A.iters_slot) *A.iters_slot = it;
+        const int it = pc_coarsest_wave(x, b, L, (double)A.omega, A.tol, A.maxit);
+        if (threadIdx.x == 0 && A.iters_slot) *A.iters_slot = it;
       }
       __syncthreads();
     } else {
@@ -384,13 +613,15 @@ __device__ __forceinline__ void coarse_vcycle_block(const PCArgs<T>& A) {
   }
   // up-leg
   for (int q = top - 1; q >= 0; q--) {
-    const Lvl<T>& L = A.lv[q];
+    const T* st = sm + A.coff[q];
+    const Lvl<T> L = pc_lvl(A.lv[q], st);
+    const ProlongW<T> P = pc_pw(A.pw[q], A.lv[q], st);
     T* x = sm + A.xoff[q];
     const T* b = sm + A.boff[q];
-    if (q + 1 < top) pc_prolong_add(x, sm + A.xoff[q + 1], L, A.lv[q + 1], A.pw[q]);
+    if (q + 1 < top) pc_prolong_add(x, sm + A.xoff[q + 1], L, A.lv[q + 1], P);
     else {
       Lvl<T> C = A.lv[top];   // geometry of the coarsest in T, compact
-      pc_prolong_add(x, sm + A.xctoff, L, C, A.pw[q]);
+      pc_prolong_add(x, sm + A.xctoff, L, C, P);
     }
     pc_smooth(x, b, L, A.omega, A.nu, false);
   }

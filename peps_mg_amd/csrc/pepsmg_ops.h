@@ -236,7 +236,7 @@ struct Ops {
       MarchGrid G = march_grid(f, 16, true, MTX);
       if (MTX == 64 || G.kchunk < 8) {
         G = march_grid(f, 16, true, 64);
-        hipLaunchKernelGGL((k_restrict_march<T, 64>), dim3((unsigned)(G.gx * G.gy * G.nch)),
+          hipLaunchKernelGGL((k_restrict_march<T, 64>), dim3((unsigned)(G.gx * G.gy * G.nch)),
                            dim3(march_threads<64>()), 0, s, c.b, x, b, f.L(), c.L(), f.RW(), G);
       } else {
         hipLaunchKernelGGL((k_restrict_march<T, MTX>), dim3((unsigned)(G.gx * G.gy * G.nch)),
@@ -284,7 +284,7 @@ struct Ops {
       MarchGrid G = prolong_grid(f, c, MTX);
       if (MTX == 64 || 2 * G.kchunk < 8) {   // chunks count coarse planes
         G = prolong_grid(f, c, 64);
-        hipLaunchKernelGGL((k_prolong_march<T, 64>), dim3((unsigned)(G.gx * G.gy * G.nch)),
+          hipLaunchKernelGGL((k_prolong_march<T, 64>), dim3((unsigned)(G.gx * G.gy * G.nch)),
                            dim3(march_threads<64>()), 0, s, xf, c.x, f.L(), c.L(), f.PW(), G);
       } else {
         hipLaunchKernelGGL((k_prolong_march<T, MTX>), dim3((unsigned)(G.gx * G.gy * G.nch)),
@@ -354,10 +354,12 @@ inline bool pc_build(int L, Hier<double>& h64, size_t lds_max, Hier<T>& h, int s
       size_t n = (size_t)d.cells();
       A.xoff[q] = (int)(take(n * sizeof(T)) / sizeof(T));
       A.boff[q] = (int)(take(n * sizeof(T)) / sizeof(T));
+      A.coff[q] = (int)(take((size_t)pc_stage_elems(d.nx, d.ny, d.nz, true) * sizeof(T)) / sizeof(T));
       A.rw[q] = d.RW();
       A.pw[q] = d.PW();
     }
   }
+  if (nlev > 1) A.roff = (int)(take((size_t)h.lv[s].cells() * sizeof(T)) / sizeof(T));
   DevLevel<double>& cd = h64.lv[L];
   Lvl<double> lc = cd.L();
   lc.px = cd.nx;
@@ -367,6 +369,7 @@ inline bool pc_build(int L, Hier<double>& h64, size_t lds_max, Hier<T>& h, int s
   A.xcoff = (int)(take(nc * 8) / 8);
   A.bcoff = (int)(take(nc * 8) / 8);
   A.xctoff = (int)(take(nc * sizeof(T)) / sizeof(T));
+  A.ccoff = (int)(take((size_t)pc_stage_elems(cd.nx, cd.ny, cd.nz, false) * 8) / 8);
   A.gl = h.lv[s].L();
   A.gxs = h.lv[s].x;
   A.gbs = h.lv[s].b;
