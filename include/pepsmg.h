@@ -119,7 +119,8 @@ int pepsmg_multigrid_solve_vcycle(pepsmg_ctx *ctx, double *sol, double *rsd, con
 
 /* Same on device arrays (padded layout) on the given HIP stream (NULL: the
  * context's own stream, ordered after the work already queued on the default
- * stream).  Returns after the stream has finished.  rsd may be NULL. */
+ * stream).  Returns after the stream has finished.  rsd may be NULL, or rhs
+ * itself (it is written after the solve). */
 int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx *ctx, double *d_sol, double *d_rsd,
                                          const double *d_rhs, int32_t maxiteration,
                                          double tolerance, double omega_sor, void *stream,

@@ -19,8 +19,9 @@
 //                          of (Ap, p) (src/cg_poisson_matrix.f90:206-207)
 //   k_resid_march<T,TY,4>: CG start r = b - A x stored in rout and in xnew (p),
 //                          partial sums of (r, r) (:196-204)
-//   k_resid_march<double,TY,6>: KIND 2, and x_in also stored into the caller's
-//       padded array (the step expected to be the last: no copy-out pass)
+//   k_resid_march<double,TY,6>: the refinement step expected to be the last:
+//       x_in = x + e stored into the caller's padded array only (no copy-out
+//       pass, no solver-layout x, no r32) and partial r^2
 //   k_refine_start:        first iterative-refinement step read straight
 //       from the caller's padded (0:n+1)^3 arrays (unaligned rows): stores x
 //       and b in the solver's layout, r as float and partial r^2 (the mixed
@@ -246,7 +247,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
         bout[u] = s0[ly[q] + 1][2 * lp[q] + 2];
         if (two) bout[u + 1] = s0[ly[q] + 1][2 * lp[q] + 3];
       }
-      if (KIND == 2 || KIND == 6) {
+      if (KIND == 2) {
         if (two) {
           V2 w;
           w.x = s0[ly[q] + 1][2 * lp[q] + 2];
@@ -280,8 +281,10 @@ k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __r
   resid_march_block<T, TY, KIND>(blockIdx.x, x, e, b, rout, r32, xnew, L, G, partial, L, (T*)nullptr);
 }
 
-// Refinement step (KIND 2) that also stores x_in into the caller's padded
-// array usr (layout U: pitches nx+2 and (nx+2)(ny+2), base at cell (1,1,1)).
+// Refinement step expected to be the last (KIND 6): x_in = x + e goes only
+// into the caller's padded array usr (layout U: pitches nx+2 and
+// (nx+2)(ny+2), base at cell (1,1,1)), with the partial sums of r^2.  xnew
+// and r32 are not written.
 template <int TY>
 __global__ void __launch_bounds__(MARCH_THREADS)
 k_refine_last(const double* __restrict__ x, const float* __restrict__ e, const double* __restrict__ b,

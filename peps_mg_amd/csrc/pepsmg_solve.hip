@@ -429,24 +429,30 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   U.pxy = (long)(f.nx + 2) * (f.ny + 2);
   const long uo = 1 + U.px + U.pxy;
   c->sol_out_done = false;
-  // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x,
-  // x2.  last: x2 also goes into the caller's array
-  auto refine_update = [&](bool last) {
-    if (last)
-      hipLaunchKernelGGL((k_refine_last<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
-                         f32.x, f.b, f32.b, f.x2, f.L(), U, G, c->d_partial, ux + uo);
-    else
-      hipLaunchKernelGGL((k_resid_march<double, RESID_TY, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0,
-                         c->stream, f.x, f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
-    std::swap(f.x, f.x2);
-  };
-  // r32 = b - A x and res0, with e = 0
-  if (ux) {   // x and b come from the caller's padded arrays and are stored in the solver's layout
+  // r32 = b - A x and its norm, x and b read from the caller's padded
+  // arrays and stored in the solver's layout (x into x2, then swap)
+  auto start_from_caller = [&]() {
     hipLaunchKernelGGL((k_refine_start<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, ux + uo,
                        ub + uo, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
     std::swap(f.x, f.x2);
+  };
+  // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x,
+  // x2.  last: x + e goes only into the caller's array, with its norm; if
+  // the step then misses the tolerance, start_from_caller restores x and r32
+  auto refine_update = [&](bool last) {
+    if (last)
+      hipLaunchKernelGGL((k_refine_last<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
+                         f32.x, f.b, (float*)nullptr, (double*)nullptr, f.L(), U, G, c->d_partial, ux + uo);
+    else
+      hipLaunchKernelGGL((k_resid_march<double, RESID_TY, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0,
+                         c->stream, f.x, f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
+    if (!last) std::swap(f.x, f.x2);
+  };
+  // r32 = b - A x and res0, with e = 0
+  if (ux) {   // x and b come from the caller's padded arrays and are stored in the solver's layout
+    start_from_caller();
   } else {
     hipMemsetAsync(f32.x, 0, f32.ncell * sizeof(float), c->stream);
     refine_update(false);
@@ -517,6 +523,7 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
       c->sol_out_done = last;
       break;
     }
+    if (last) start_from_caller();   // mispredicted: x + e is in the caller's array only
     if (rho > 0.0 && rho_prev > 0.0) fcyc = std::pow(rho / rho_prev, 1.0 / nin);
   }
   if (ncyc > 0) {
@@ -656,6 +663,8 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
   if (!rc) {
     if (!(direct && c->sol_out_done)) unpack(c->stream, d_sol, f.x, f, 0);   // else the last step stored it
     if (d_rsd) {
+      // a last refinement step stored x + e into sol only: take x from there
+      if (direct && c->sol_out_done) pack(c->stream, f.x, d_sol, f, (int*)nullptr);
       Ops<double> op{c->stream};
       op.residual(f, f.r, f.x, f.b);                                   // :1970
       unpack(c->stream, d_rsd, f.r, f, 1);
@@ -686,19 +695,12 @@ int pepsmg_multigrid_solve_vcycle(pepsmg_ctx* c, double* sol, double* rsd, const
   }
   HIP_TRY(hipMemcpy(c->d_pad, sol, np * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_pad2, rhs, np * sizeof(double), hipMemcpyHostToDevice));
-  // rhs staging is reused for rsd after the solve
-  rc = pepsmg_multigrid_solve_vcycle_device(c, c->d_pad, nullptr, c->d_pad2, maxiteration, tolerance, omega_sor,
-                                            nullptr, st);
+  // the rhs staging receives rsd after the solve (b is in the solver's layout by then)
+  rc = pepsmg_multigrid_solve_vcycle_device(c, c->d_pad, rsd ? c->d_pad2 : nullptr, c->d_pad2, maxiteration,
+                                            tolerance, omega_sor, nullptr, st);
   if (rc) return rc;
   HIP_TRY(hipMemcpy(sol, c->d_pad, np * sizeof(double), hipMemcpyDeviceToHost));
-  if (rsd) {
-    DevLevel<double>& f = c->h64.lv[0];
-    Ops<double> op{c->stream};
-    op.residual(f, f.r, f.x, f.b);
-    unpack(c->stream, c->d_pad2, f.r, f, 1);
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(rsd, c->d_pad2, np * sizeof(double), hipMemcpyDeviceToHost));
-  }
+  if (rsd) HIP_TRY(hipMemcpy(rsd, c->d_pad2, np * sizeof(double), hipMemcpyDeviceToHost));
   return PEPSMG_OK;
 }
 

@@ -171,6 +171,40 @@ def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty, inner):
     assert err < 1e-8, err
 
 
+def test_mixed_refinement_end_states():
+    """Mixed precision over a range of tolerances.  The refinement step
+    predicted to be the last stores x + e only into sol; when it misses the
+    tolerance, x and the FP32 residual are restored from sol and the solve
+    goes on.  Across the sweep both outcomes occur; in every case the
+    returned sol, rsd (host and device entry points, rsd aliasing rhs on the
+    device) and rho agree with an independent FP64 evaluation of b - A x
+    (scipy sparse Laplacian)."""
+    import torch
+    n = 48
+    g = O.Grid(n, n, n)
+    b = -O.uniform_field(n, n, n, 5)
+    K = RM.laplacian(g.dxm[1:-1], g.dym[1:-1], g.dzm[1:-1])
+    mg = Multigrid(sdm_from(g), 3, 80, precision=1)
+    bmax = np.abs(b).max()
+    steps = set()
+    for tol in np.geomspace(1e-6, 3e-13, 15):
+        xg, rg = O.padded(n, n, n), O.padded(n, n, n)
+        res = mg.solve_vcycle(xg, rg, None, b, 2, tol, 1.0)
+        assert res.converged and res.rho < tol
+        r = RM.flat(b) - K @ RM.flat(xg)
+        assert abs(np.linalg.norm(r) / np.sqrt(res.res0) - res.rho) <= 1e-3 * res.rho + 1e-15
+        assert np.abs(RM.flat(rg) - r).max() <= 1e-11 * bmax
+        assert not rg[0].any() and not rg[-1].any() and not rg[:, 0].any() and not rg[:, :, -1].any()
+        steps.add(res.refine_steps)
+        d_b = torch.from_numpy(b).cuda()
+        d_x = torch.zeros_like(d_b)
+        res_d = mg.solve_vcycle_device(d_x.data_ptr(), d_b.data_ptr(), d_b.data_ptr(), 2, tol, 1.0)
+        torch.cuda.synchronize()
+        assert res_d.refine_steps == res.refine_steps and res_d.rho == res.rho
+        assert np.array_equal(d_x.cpu().numpy(), xg) and np.array_equal(d_b.cpu().numpy(), rg)
+    assert len(steps) >= 3, steps   # the sweep spans several step counts
+
+
 @pytest.mark.parametrize("precision,where", [(0, (0, 3, 3)), (1, (0, 3, 3)), (1, (3, 9, 3)), (1, (3, 3, 9)),
                                              (1, (9, 9, 9))])
 def test_ghost_cells_rejected(precision, where):
