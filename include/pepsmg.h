@@ -73,6 +73,8 @@ typedef struct pepsmg_options {
   double inner_omega;      /* precision 1: SOR factor of the FP32 V-cycles' red/black smoothing, in (0, 2)
                               (default 1.35; 0: the caller's omega_sor); the FP64 residual and update,
                               which decide convergence, do not use it */
+  int32_t inner_nu_post;   /* precision 1: post-smoothing steps of the FP32 V-cycles (default 0: the same
+                              as inner_nu, which then counts pre-smoothing steps only) */
 } pepsmg_options;
 
 /* Per-solve statistics (the reference only prints these, src/multigrid.f90:1972). */

@@ -33,7 +33,8 @@ class Options(C.Structure):
     _fields_ = [("smoother", C.c_int32), ("precision", C.c_int32), ("coarse_switch", C.c_int32),
                 ("inner_cycles", C.c_int32), ("max_refine", C.c_int32), ("fused_smoother", C.c_int32),
                 ("tile_y", C.c_int32), ("verbose", C.c_int32), ("tile_x", C.c_int32),
-                ("inner_nu", C.c_int32), ("inner_omega", C.c_double)]
+                ("inner_nu", C.c_int32), ("inner_omega", C.c_double),
+                ("inner_nu_post", C.c_int32)]
 
 
 def option_value(name, v):

@@ -6,9 +6,9 @@ Everything is measured on the GPU through the C-ABI; nothing is modelled:
   (``pepsmg_bench_smoother``, HIP events) for each tile shape and prefetch
   depth and returns the fastest;
 * ``tune_solver`` times complete solves over a small space of schedule
-  options (FP32 V-cycles per refinement step, their smoothing steps and
-  SOR factor, size of the levels handed to the persistent coarse kernel)
-  with the tuned tile;
+  options (FP32 V-cycles per refinement step, their pre- and post-smoothing
+  steps and SOR factor, size of the levels handed to the persistent coarse
+  kernel) with the tuned tile;
 * ``tune_batch`` gives every problem of a batch the tile tuned for its grid
   size (one measurement per distinct size) and then picks the schedule
   options that make the whole batch fastest.
@@ -92,27 +92,28 @@ def _opts(d):
 
 
 def tune_solver(sdm, nlevel, d_sol, d_rhs, maxiteration=2, tolerance=1e-10, omega=1.0, precision=1,
-                inner_cycles=(2, 3), inner_nu=(1, 2), inner_omega=(1.2, 1.35), coarse_switch=(512, 4096, 32768),
-                repeats=2, use_cache=True):
+                inner_cycles=(2, 3), inner_nu=(1, 2), inner_nu_post=(1, 2), inner_omega=(1.2, 1.35),
+                coarse_switch=(512, 4096, 32768), repeats=2, use_cache=True):
     """Fastest schedule options for complete solves of this grid; d_sol /
     d_rhs are padded device arrays (torch tensors; d_sol is zeroed before
     each timed solve).  Returns the option dict (tile options included)."""
     import torch
     from .multigrid import Multigrid
     db = _load() if use_cache else {}
-    key = _key("solve", sdm, precision, f":nl{nlevel}:nu{maxiteration}")
+    key = _key("solve", sdm, precision, f":nl{nlevel}:nu{maxiteration}:v2")
     if key in db:
         return db[key]
     tiles = _opts(tune_tiles(sdm, nlevel, precision, use_cache=use_cache))
     mg = Multigrid(sdm, nlevel, 200, precision=precision, **tiles)
     best, best_t, timings = None, float("inf"), {}
     dflt = _lib.default_options()
-    sched = ([(ic, nu, om) for ic in inner_cycles for nu in inner_nu for om in inner_omega] if precision == 1
-             else [(dflt.inner_cycles, dflt.inner_nu, dflt.inner_omega)])
+    sched = ([(ic, nu, nup, om) for ic in inner_cycles for nu in inner_nu for nup in inner_nu_post
+              for om in inner_omega] if precision == 1
+             else [(dflt.inner_cycles, dflt.inner_nu, dflt.inner_nu_post, dflt.inner_omega)])
     for cs in coarse_switch:
-        for ic, nui, om in sched:
+        for ic, nui, nup, om in sched:
             try:
-                mg.set_options(coarse_switch=cs, inner_cycles=ic, inner_nu=nui, inner_omega=om)
+                mg.set_options(coarse_switch=cs, inner_cycles=ic, inner_nu=nui, inner_nu_post=nup, inner_omega=om)
             except _lib.PepsmgError:
                 continue
             t = float("inf")
@@ -124,10 +125,10 @@ def tune_solver(sdm, nlevel, d_sol, d_rhs, maxiteration=2, tolerance=1e-10, omeg
                 torch.cuda.synchronize()
                 if r.converged:
                     t = min(t, time.perf_counter() - t0)
-            timings[f"cs{cs}/ic{ic}/nu{nui}/om{om}"] = t
+            timings[f"cs{cs}/ic{ic}/nu{nui}-{nup}/om{om}"] = t
             if t < best_t:
-                best_t, best = t, dict(tiles, coarse_switch=cs, inner_cycles=ic, inner_nu=nui, inner_omega=om,
-                                       precision=precision)
+                best_t, best = t, dict(tiles, coarse_switch=cs, inner_cycles=ic, inner_nu=nui, inner_nu_post=nup,
+                                       inner_omega=om, precision=precision)
     mg.destroy()
     best = dict(best, solve_s=best_t, timings=timings)
     if use_cache:
@@ -138,11 +139,11 @@ def tune_solver(sdm, nlevel, d_sol, d_rhs, maxiteration=2, tolerance=1e-10, omeg
 
 
 def tune_batch(batch, sols, rhss, maxiteration=2, tolerance=1e-10, omega=1.0, inner_cycles=(2, 3),
-               inner_nu=(1, 2), inner_omega=(1.2, 1.35), repeats=1, use_cache=True):
+               inner_nu=(1, 2), inner_nu_post=(1, 2), inner_omega=(1.2, 1.35), repeats=1, use_cache=True):
     """Per-problem tiles by grid size, then the inner-cycle schedule that
     makes the whole batch fastest; applies both to `batch` (a BatchSolver)
     and returns {'tiles': {size: tile}, 'inner_cycles': n, 'inner_nu': n,
-    'inner_omega': w, 'timings': {...}}."""
+    'inner_nu_post': n, 'inner_omega': w, 'timings': {...}}."""
     import torch
     precision = batch.options.precision
     tiles_by_size = {}
@@ -156,10 +157,12 @@ def tune_batch(batch, sols, rhss, maxiteration=2, tolerance=1e-10, omega=1.0, in
         ty.append(t["tile_y"])
     batch.set_tiles(tx, ty)
     timings, best, best_t = {}, None, float("inf")
-    sched = ([(ic, nu, om) for ic in inner_cycles for nu in inner_nu for om in inner_omega] if precision == 1
-             else [(batch.options.inner_cycles, batch.options.inner_nu, batch.options.inner_omega)])
-    for ic, nui, om in sched:
-        batch.set_options(inner_cycles=ic, inner_nu=nui, inner_omega=om)
+    o = batch.options
+    sched = ([(ic, nu, nup, om) for ic in inner_cycles for nu in inner_nu for nup in inner_nu_post
+              for om in inner_omega] if precision == 1
+             else [(o.inner_cycles, o.inner_nu, o.inner_nu_post, o.inner_omega)])
+    for ic, nui, nup, om in sched:
+        batch.set_options(inner_cycles=ic, inner_nu=nui, inner_nu_post=nup, inner_omega=om)
         t = float("inf")
         for _ in range(repeats + 1):
             for x in sols:
@@ -170,10 +173,11 @@ def tune_batch(batch, sols, rhss, maxiteration=2, tolerance=1e-10, omega=1.0, in
             torch.cuda.synchronize()
             if all(r.converged for r in res):
                 t = min(t, time.perf_counter() - t0)
-        timings[f"ic{ic}/nu{nui}/om{om}"] = t
+        timings[f"ic{ic}/nu{nui}-{nup}/om{om}"] = t
         if t < best_t:
-            best_t, best = t, (ic, nui, om)
-    batch.set_options(inner_cycles=best[0], inner_nu=best[1], inner_omega=best[2])
+            best_t, best = t, (ic, nui, nup, om)
+    batch.set_options(inner_cycles=best[0], inner_nu=best[1], inner_nu_post=best[2], inner_omega=best[3])
     return {"tiles": {f"{k[0]}x{k[1]}x{k[2]}": {"tile_x": v["tile_x"], "tile_y": v["tile_y"]}
                       for k, v in tiles_by_size.items()},
-            "inner_cycles": best[0], "inner_nu": best[1], "inner_omega": best[2], "timings": timings}
+            "inner_cycles": best[0], "inner_nu": best[1], "inner_nu_post": best[2], "inner_omega": best[3],
+            "timings": timings}

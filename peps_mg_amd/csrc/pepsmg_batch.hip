@@ -420,13 +420,14 @@ static void rec_prolong(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int
 }
 
 template <typename T>
-static void rec_coarse(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int nu, T omega, double tol) {
+static void rec_coarse(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int nu, int nu_post, T omega,
+                       double tol) {
   std::vector<PCArgs<T>> tab(B->nitems);
   size_t lds = 0;
   for (int i = 0; i < B->nitems; i++) {
     BItem& it = B->items[i];
     size_t li = 0;
-    pc_build(it.nlevel, it.h64, B->lds_max, *H[i], it.s, nu, omega, tol, (int*)nullptr, tab[i], li);
+    pc_build(it.nlevel, it.h64, B->lds_max, *H[i], it.s, nu, nu_post, omega, tol, (int*)nullptr, tab[i], li);
     lds = std::max(lds, li);
   }
   size_t off = S.put(tab);
@@ -439,18 +440,18 @@ static void rec_coarse(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int 
 
 // one V-cycle over every item of hierarchy set H (src/multigrid.f90:1931-1967)
 template <typename T>
-static void rec_vcycle(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int nu, T omega, double tol,
-                       bool fine_zero) {
+static void rec_vcycle(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H, int nu, int nu_post, T omega,
+                       double tol, bool fine_zero) {
   int D = 0;
   for (auto& it : B->items) D = std::max(D, it.s);
   for (int l = 0; l < D; l++) {
     rec_smooth<T>(B, S, H, l, nu, omega, l > 0 || fine_zero);
     rec_restrict<T>(B, S, H, l);
   }
-  rec_coarse<T>(B, S, H, nu, omega, tol);
+  rec_coarse<T>(B, S, H, nu, nu_post, omega, tol);
   for (int l = D - 1; l >= 0; l--) {
     rec_prolong<T>(B, S, H, l);
-    rec_smooth<T>(B, S, H, l, nu, omega, false);
+    rec_smooth<T>(B, S, H, l, nu_post, omega, false);
   }
 }
 
@@ -602,7 +603,7 @@ int pepsmg_batch_set_options(pepsmg_batch* B, const pepsmg_options* opt) {
   if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16 && opt->tile_y != 32)
     return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8, 16 or 32");
   if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
-  if (opt->inner_nu < 0) return fail(PEPSMG_ERR_ARG, "inner_nu must be >= 0");
+  if (opt->inner_nu < 0 || opt->inner_nu_post < 0) return fail(PEPSMG_ERR_ARG, "inner_nu/inner_nu_post must be >= 0");
   if (!(opt->inner_omega >= 0.0 && opt->inner_omega < 2.0))
     return fail(PEPSMG_ERR_ARG, "inner_omega must be 0 (caller's omega) or in (0, 2)");
   if (opt->inner_cycles < 1 || opt->max_refine < 1) return fail(PEPSMG_ERR_ARG, "inner_cycles/max_refine must be >= 1");
@@ -655,11 +656,11 @@ static int batch_switch_levels(pepsmg_batch* B, int nu, double omega, double tol
       if (B->opt.precision == 1) {
         PCArgs<float> A;
         ok = it.h32.lv[l].cells() <= B->opt.coarse_switch &&
-             pc_build(it.nlevel, it.h64, B->lds_max, it.h32, l, nu, (float)omega, tol, (int*)nullptr, A, lds);
+             pc_build(it.nlevel, it.h64, B->lds_max, it.h32, l, nu, nu, (float)omega, tol, (int*)nullptr, A, lds);
       } else {
         PCArgs<double> A;
         ok = it.h64.lv[l].cells() <= B->opt.coarse_switch &&
-             pc_build(it.nlevel, it.h64, B->lds_max, it.h64, l, nu, omega, tol, (int*)nullptr, A, lds);
+             pc_build(it.nlevel, it.h64, B->lds_max, it.h64, l, nu, nu, omega, tol, (int*)nullptr, A, lds);
       }
       if (ok) {
         it.s = l;
@@ -694,8 +695,9 @@ static int batch_build(pepsmg_batch* B, int nu, double tol, double omega) {
     for (int p = 0; p < 2; p++) {
       Sched& S = B->sched[p];
       const int nui = B->opt.inner_nu > 0 ? B->opt.inner_nu : nu;
+      const int nup = B->opt.inner_nu_post > 0 ? B->opt.inner_nu_post : nui;
       const float omi = (float)(B->opt.inner_omega > 0.0 ? B->opt.inner_omega : omega);
-      for (int q = 0; q < B->opt.inner_cycles; q++) rec_vcycle<float>(B, S, H, nui, omi, tol, q == 0);
+      for (int q = 0; q < B->opt.inner_cycles; q++) rec_vcycle<float>(B, S, H, nui, nup, omi, tol, q == 0);
       rec_resid<2>(B, S, true);
       rec_check(B, S, tol, maxsteps, 0);
     }
@@ -703,7 +705,7 @@ static int batch_build(pepsmg_batch* B, int nu, double tol, double omega) {
     std::vector<Hier<double>*> H;
     for (auto& it : B->items) H.push_back(&it.h64);
     Sched& S = B->sched[0];
-    rec_vcycle<double>(B, S, H, nu, omega, tol, false);
+    rec_vcycle<double>(B, S, H, nu, nu, omega, tol, false);
     rec_resid<0>(B, S, false);
     rec_check(B, S, tol, maxsteps, 0);
   }

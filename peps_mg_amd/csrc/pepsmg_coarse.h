@@ -194,7 +194,7 @@ constexpr int PC_THREADS = 1024;
 template <typename T>
 struct PCArgs {
   int nlev;                       // levels s..L handled: nlev = L - s + 1
-  int nu;
+  int nu, nu_post;                // smoothing steps on the way down, on the way up
   T omega;
   double tol;
   int maxit;
@@ -623,7 +623,7 @@ __device__ __forceinline__ void coarse_vcycle_block(const PCArgs<T>& A) {
       Lvl<T> C = A.lv[top];   // geometry of the coarsest in T, compact
       pc_prolong_add(x, sm + A.xctoff, L, C, P);
     }
-    pc_smooth(x, b, L, A.omega, A.nu, false);
+    pc_smooth(x, b, L, A.omega, A.nu_post, false);
   }
   // x_s to global
   {

@@ -315,6 +315,7 @@ void pepsmg_default_options(pepsmg_options* o) {
   o->tile_x = 0;
   o->inner_nu = 2;
   o->inner_omega = 1.35;
+  o->inner_nu_post = 0;
 }
 
 int pepsmg_multigrid_create(const pepsmg_grid* grid, int32_t nlevel, int32_t ncycle, int32_t aggr_method,
@@ -363,7 +364,7 @@ int pepsmg_set_options(pepsmg_ctx* ctx, const pepsmg_options* opt) {
   if (opt->tile_y != 0 && opt->tile_y != 8 && opt->tile_y != 16 && opt->tile_y != 32)
     return fail(PEPSMG_ERR_ARG, "tile_y must be 0, 8, 16 or 32");
   if (opt->tile_x != 0 && opt->tile_x != 64 && opt->tile_x != 128) return fail(PEPSMG_ERR_ARG, "tile_x must be 0, 64 or 128");
-  if (opt->inner_nu < 0) return fail(PEPSMG_ERR_ARG, "inner_nu must be >= 0");
+  if (opt->inner_nu < 0 || opt->inner_nu_post < 0) return fail(PEPSMG_ERR_ARG, "inner_nu/inner_nu_post must be >= 0");
   if (!(opt->inner_omega >= 0.0 && opt->inner_omega < 2.0))
     return fail(PEPSMG_ERR_ARG, "inner_omega must be 0 (caller's omega) or in (0, 2)");
   ctx->opt = *opt;

@@ -327,13 +327,15 @@ struct Ops {
 // compact LDS layout of every level's x and b (16-byte aligned), the
 // coarsest level in FP64.  Returns false if it does not fit in LDS.
 template <typename T>
-inline bool pc_build(int L, Hier<double>& h64, size_t lds_max, Hier<T>& h, int s, int nu, T omega, double tol,
+inline bool pc_build(int L, Hier<double>& h64, size_t lds_max, Hier<T>& h, int s, int nu, int nu_post, T omega,
+                     double tol,
                      int* slot, PCArgs<T>& A, size_t& lds) {
   int nlev = L - s + 1;
   if (s < 1 || nlev > PC_MAXLEV) return false;
   std::memset(&A, 0, sizeof A);
   A.nlev = nlev;
   A.nu = nu;
+  A.nu_post = nu_post;
   A.omega = omega;
   A.tol = tol;
   A.maxit = 1000;

@@ -258,7 +258,8 @@ static void coarse_solve_level(pepsmg_ctx* c, Hier<float>& h, float omega, doubl
 // in the persistent single-workgroup kernel (RBGS smoother only); the
 // coarsest solve is FP64 in every precision.
 template <typename T>
-static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int nu, T omega, double tol,
+static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int nu, int nu_post, T omega,
+                               double tol,
                                int* slot) {
   int L = c->nlevel;
   int s = L + 1;
@@ -267,7 +268,7 @@ static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>*
   if (c->opt.smoother == 0 && c->opt.coarse_switch > 0) {
     for (int l = 1; l <= L; l++) {
       if (h.lv[l].cells() <= c->opt.coarse_switch &&
-          pc_build(c->nlevel, c->h64, c->lds_max, h, l, nu, omega, tol, slot, A, lds)) {
+          pc_build(c->nlevel, c->h64, c->lds_max, h, l, nu, nu_post, omega, tol, slot, A, lds)) {
         s = l;
         break;
       }
@@ -286,7 +287,7 @@ static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>*
   } else {
     coarse_solve_level(c, h, omega, tol, slot);
   }
-  for (int l = last; l >= 1; l--) smooth_level(c, op, h, cg, l, nu, omega, false, true);
+  for (int l = last; l >= 1; l--) smooth_level(c, op, h, cg, l, nu_post, omega, false, true);
 }
 
 static void print_cycle(int cyc, double rv, double r0) {
@@ -375,7 +376,7 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
     }
     op.residual_restrict(f, h.lv[1], f.x, f.b);                            // :1932-1933
     int* slot = (cyc - 1 < c->citers_cap) ? c->d_citers + (cyc - 1) : nullptr;
-    vcycle_coarse_part<double>(c, op, h, cg, nu, omega, tol, slot);        // :1936-1962
+    vcycle_coarse_part<double>(c, op, h, cg, nu, nu, omega, tol, slot);    // :1936-1962
     smooth_level(c, op, h, cg, 0, nu, omega, false, true);                 // :1965-1967
     if (!fold || cyc == c->ncycle) {
       op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 1);  // :1970-1971
@@ -460,9 +461,11 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   // SOR factor of the FP32 inner cycles (an inner-solver setting: the FP64
   // residual and update, which decide convergence, do not use it)
   float om = (float)(c->opt.inner_omega > 0.0 ? c->opt.inner_omega : omega);
-  // smoothing steps of the FP32 V-cycles (the FP64 residual and update do
-  // not depend on them): inner_nu, or the caller's maxiteration
+  // pre- and post-smoothing steps of the FP32 V-cycles (the FP64 residual
+  // and update do not depend on them): inner_nu (or the caller's
+  // maxiteration) and inner_nu_post (or the pre-smoothing count)
   const int nui = c->opt.inner_nu > 0 ? c->opt.inner_nu : nu;
+  const int nup = c->opt.inner_nu_post > 0 ? c->opt.inner_nu_post : nui;
   // The host reads each norm while the GPU already runs the next step's
   // first pre-smoothing (it needs only r32, and touches no FP64 array), so
   // the read-back leaves no gap; skipped when the step is predicted to stop.
@@ -503,8 +506,8 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
       int* slot = (ncyc < c->citers_cap) ? c->d_citers + ncyc : nullptr;
       if (!(q == 0 && ahead)) smooth_level(c, op, g, cg, 0, nui, om, q == 0, false);
       op.residual_restrict(f32, g.lv[1], f32.x, f32.b);
-      vcycle_coarse_part<float>(c, op, g, cg, nui, om, tol, slot);
-      smooth_level(c, op, g, cg, 0, nui, om, false, true);
+      vcycle_coarse_part<float>(c, op, g, cg, nui, nup, om, tol, slot);
+      smooth_level(c, op, g, cg, 0, nup, om, false, true);
       ncyc++;
     }
     // predicted to stop here: the previous per-cycle factor over this step's cycles, factor-2 margin

@@ -18,6 +18,7 @@ module pepsmg_c
         integer(c_int32_t) :: max_refine, fused_smoother, tile_y, verbose
         integer(c_int32_t) :: tile_x, inner_nu
         real(c_double)     :: inner_omega
+        integer(c_int32_t) :: inner_nu_post
     end type pepsmg_options_t
 
     !> struct pepsmg_stats
