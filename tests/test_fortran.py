@@ -159,3 +159,29 @@ def test_driver_writes_reference_solution_files(tmp_path):
     last = [float(v.replace("D", "E")) for v in lines[-1].split()]
     assert abs(last[3] - x[-1, -1, -1]) <= 1e-18 + 1e-19 * abs(x[-1, -1, -1])
     assert len(lines[0]) == 3 * 20 + 30
+
+
+def test_bind_c_types_match_the_c_structs(tmp_path):
+    """The Fortran bind(C) mirrors of pepsmg_grid, pepsmg_options and
+    pepsmg_stats have the sizes of the C structs (whose field offsets
+    test_abi checks against a C compiler)."""
+    import ctypes as C
+    from peps_mg_amd import _lib
+    if not os.path.exists(fortran_build.FC):
+        pytest.skip("no Fortran compiler")
+    ensure_built()
+    src = tmp_path / "layout.f90"
+    src.write_text("program layout\n"
+                   "    use iso_c_binding\n"
+                   "    use pepsmg_c\n"
+                   "    type(pepsmg_grid_t) :: g\n"
+                   "    type(pepsmg_options_t) :: o\n"
+                   "    type(pepsmg_stats_t) :: s\n"
+                   "    print '(3i8)', c_sizeof(g), c_sizeof(o), c_sizeof(s)\n"
+                   "end program layout\n")
+    exe = tmp_path / "layout"
+    subprocess.run([fortran_build.FC, "-I", fortran_build.BUILD, str(src), "-o", str(exe)], check=True,
+                   cwd=tmp_path)
+    sizes = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                            text=True).stdout.split()]
+    assert sizes == [C.sizeof(_lib.Grid), C.sizeof(_lib.Options), C.sizeof(_lib.Stats)]
