@@ -78,25 +78,30 @@ int build_hierarchy(const pepsmg_grid* g, int nlevel, int aggr, int cga_level, s
 // Extra per-level arrays of the CG smoother (p and A p).
 template <typename T>
 struct CgBufs {
-  std::vector<T*> p, ap;
+  std::vector<T*> p, p2, ap;   // direction, the next direction, A p
   int ensure(Hier<T>& h) {
     if (!p.empty()) return 0;
     for (auto& d : h.lv) {
-      T *a = nullptr, *b = nullptr;
+      T *a = nullptr, *b = nullptr, *c = nullptr;
       hipError_t e = hipMalloc(&a, d.ncell * sizeof(T));
       if (e == hipSuccess) e = hipMalloc(&b, d.ncell * sizeof(T));
+      if (e == hipSuccess) e = hipMalloc(&c, d.ncell * sizeof(T));
       if (e != hipSuccess) return hip_fail(e, "hipMalloc (CG buffers)");
       hipMemset(a, 0, d.ncell * sizeof(T));
       hipMemset(b, 0, d.ncell * sizeof(T));
+      hipMemset(c, 0, d.ncell * sizeof(T));
       p.push_back(a);
       ap.push_back(b);
+      p2.push_back(c);
     }
     return 0;
   }
   void release() {
     for (auto q : p) hipFree(q);
+    for (auto q : p2) hipFree(q);
     for (auto q : ap) hipFree(q);
     p.clear();
+    p2.clear();
     ap.clear();
   }
 };

@@ -15,6 +15,9 @@
 //   k_resid_march<T,TY,1>: same, and stores r
 //   k_resid_march<double,TY,2>: iterative-refinement step: x_in = x + e
 //       (e the FP32 correction), stores x_in, r as float, partial r^2
+//   k_cg_ap_fold<T,TY> (KIND 7): KIND 3 on pnew = r + beta p, formed while
+//                          staging; pnew stored too (the direction update
+//                          of the previous CG iteration, folded in)
 //   k_resid_march<T,TY,3>: CG step Ap = A p (stored in rout) and partial sums
 //                          of (Ap, p) (src/cg_poisson_matrix.f90:206-207)
 //   k_resid_march<T,TY,4>: CG start r = b - A x stored in rout and in xnew (p),
@@ -43,16 +46,20 @@ struct MarchGrid {
 // Stages x planes (4 slots) of a TX x TY tile: x = i0-2 .. i0+TX+1, y = j0-1 .. j0+TY.
 // issue() loads one plane into registers from clamped in-bounds addresses
 // with no branch around the loads; stage() zeroes what lies outside the
-// domain, adds the FP32 correction (MIXED) and writes LDS.  Keeping every
-// select away from the loads lets the compiler wait only for the loads a
-// plane needs instead of draining the ones in flight.
-template <typename T, int TY, bool MIXED, bool UNAL = false, int TX = 64, int NT = MARCH_THREADS>
+// domain, adds the FP32 correction (MIXED) or forms the CG direction
+// x + beta * cgp (CGP: x is r, cgp the previous direction) and writes LDS.
+// Keeping every select away from the loads lets the compiler wait only for
+// the loads a plane needs instead of draining the ones in flight.
+template <typename T, int TY, bool MIXED, bool UNAL = false, int TX = 64, int NT = MARCH_THREADS, bool CGP = false>
 struct XStager {
   static constexpr int RX = TX + 4, RY = TY + 2, XP = RX / 2;
   static constexpr int NXL = (RY * XP + NT - 1) / NT;
   using V2 = typename Vec2<T>::type;
   V2 pre[NXL];
   float2 epre[MIXED ? NXL : 1];
+  V2 ppre[CGP ? NXL : 1];
+  const T* cgp = nullptr;
+  T beta = T(0);
   int mq;
 
   __device__ __forceinline__ void issue(const T* __restrict__ x, const float* __restrict__ e, const Lvl<T>& L, int i0,
@@ -75,6 +82,7 @@ struct XStager {
         pre[q] = *reinterpret_cast<const V2*>(x + q0);
       }
       if (MIXED) epre[q] = *reinterpret_cast<const float2*>(e + q0);
+      if (CGP) ppre[q] = *reinterpret_cast<const V2*>(cgp + q0);
     }
   }
   __device__ __forceinline__ void stage(T (*s)[RX], const Lvl<T>& L, int i0, int j0) {
@@ -92,6 +100,10 @@ struct XStager {
         if (MIXED) {
           a = a + (T)epre[q].x;
           b = b + (T)epre[q].y;
+        }
+        if (CGP) {   // p = r + beta p (src/cg_poisson_matrix.f90:229)
+          a = a + beta * ppre[q].x;
+          b = b + beta * ppre[q].y;
         }
         s[ry][2 * px] = va ? a : T(0);
         s[ry][2 * px + 1] = vb ? b : T(0);
@@ -125,10 +137,12 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
                                                   const T* __restrict__ b, T* __restrict__ rout,
                                                   float* __restrict__ r32, T* __restrict__ xnew, const Lvl<T>& L,
                                                   const MarchGrid& G, double* __restrict__ partial,
-                                                  const Lvl<T>& LI, T* __restrict__ bout) {
+                                                  const Lvl<T>& LI, T* __restrict__ bout, T beta = T(0)) {
   constexpr bool MIXED = (KIND == 2 || KIND == 6);
   constexpr bool UNAL = (KIND == 5);
-  using St = XStager<T, TY, MIXED, UNAL>;
+  constexpr bool CGP = (KIND == 7);   // x is r, b the previous direction p
+  constexpr bool AP = (KIND == 3 || KIND == 7);
+  using St = XStager<T, TY, MIXED, UNAL, 64, MARCH_THREADS, CGP>;
   constexpr int RX = St::RX, RY = St::RY;
   constexpr int NP = (TY * 32 + MARCH_THREADS - 1) / MARCH_THREADS;   // cell pairs per thread
   using V2 = typename Vec2<T>::type;
@@ -166,7 +180,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
   // b of this thread's pairs, from clamped addresses (values of cells
   // outside the domain are never used)
   auto issue_b = [&](int m) {
-    if (KIND == 3) return;                   // A p needs no right-hand side
+    if (AP) return;                          // A p needs no right-hand side
     const int mc = min(max(m, 1), L.nz);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
@@ -182,6 +196,10 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
   };
 
   const Lvl<T>& LX = UNAL ? LI : L;   // layout of the x input
+  if (CGP) {
+    st.cgp = b;
+    st.beta = beta;
+  }
   st.issue(x, e, LX, i0, j0, k0 - 1);
   st.stage(sx[(k0 - 1) & 3], L, i0, j0);
   st.issue(x, e, LX, i0, j0, k0);
@@ -210,7 +228,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
       int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
       bool two = gi + 1 <= L.nx;
       T ra, rb;
-      if (KIND == 3) {
+      if (AP) {
         ra = march_apply<T, RX>(s0, sm, sp, 2 * lp[q], ly[q], c1a[q], c2a[q], c3[q], c4[q], c5, c6);
         rb = two ? march_apply<T, RX>(s0, sm, sp, 2 * lp[q] + 1, ly[q], c1b[q], c2b[q], c3[q], c4[q], c5, c6) : T(0);
         acc += (double)ra * (double)s0[ly[q] + 1][2 * lp[q] + 2];
@@ -223,9 +241,13 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
         acc += (double)rb * (double)rb;
       }
       long q0 = lidx(L, gi, gj, m);
-      if (KIND == 1 || KIND == 3 || KIND == 4) {
+      if (KIND == 1 || AP || KIND == 4) {
         if (two) st_stream_pair(rout + q0, ra, rb);
         else rout[q0] = ra;
+      }
+      if (KIND == 7) {   // the new direction p of this tile's cells
+        if (two) st_stream_pair(xnew + q0, s0[ly[q] + 1][2 * lp[q] + 2], s0[ly[q] + 1][2 * lp[q] + 3]);
+        else xnew[q0] = s0[ly[q] + 1][2 * lp[q] + 2];
       }
       if (KIND == 4) {
         if (two) st_stream_pair(xnew + q0, ra, rb);
@@ -279,6 +301,20 @@ __global__ void __launch_bounds__(MARCH_THREADS)
 k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __restrict__ b, T* __restrict__ rout,
               float* __restrict__ r32, T* __restrict__ xnew, Lvl<T> L, MarchGrid G, double* __restrict__ partial) {
   resid_march_block<T, TY, KIND>(blockIdx.x, x, e, b, rout, r32, xnew, L, G, partial, L, (T*)nullptr);
+}
+
+// CG step with the direction update folded in (KIND 7): pnew = r + beta p
+// (beta = sc[new_slot] / sc[old_slot], src/cg_poisson_matrix.f90:223-229)
+// formed while staging, stored for the tile cells, Ap = A pnew stored in
+// ap, partial sums of (Ap, pnew) (:206-207).  p and pnew are different
+// arrays (neighbouring tiles still read p).
+template <typename T, int TY>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_cg_ap_fold(const T* __restrict__ r, const T* __restrict__ p, T* __restrict__ pnew, T* __restrict__ ap, Lvl<T> L,
+             MarchGrid G, double* __restrict__ partial, const double* __restrict__ sc, int new_slot, int old_slot) {
+  const T beta = (T)(sc[new_slot] / sc[old_slot]);
+  resid_march_block<T, TY, 7>(blockIdx.x, r, (const float*)nullptr, p, ap, (float*)nullptr, pnew, L, G, partial, L,
+                              (T*)nullptr, beta);
 }
 
 // Refinement step expected to be the last (KIND 6): x_in = x + e goes only

@@ -99,8 +99,11 @@ k_cg_p(T* __restrict__ p, const T* __restrict__ r, const double* __restrict__ sc
   });
 }
 
+// p2: a second direction buffer (with the march kernels the direction
+// update of iteration it-1 is folded into iteration it's A p launch, which
+// writes the new direction there; p and p2 then trade places)
 template <typename T>
-static void cg_iterate(Ops<T>& op, DevLevel<T>& d, T* x, const T* b, T* r, T* p, T* ap, int nu, bool zero_x,
+static void cg_iterate(Ops<T>& op, DevLevel<T>& d, T* x, const T* b, T* r, T* p, T* p2, T* ap, int nu, bool zero_x,
                        double* partial, double* sc) {
   const long nrow = (long)d.ny * d.nz;
   const int eb = (int)std::max(1L, std::min<long>((nrow + CG_WAVES - 1) / CG_WAVES, RED_BLOCKS));
@@ -126,7 +129,12 @@ static void cg_iterate(Ops<T>& op, DevLevel<T>& d, T* x, const T* b, T* r, T* p,
   }
   for (int it = 0; it < nu; it++) {
     const int cur = it & 1, nxt = (it + 1) & 1;
-    if (march) {                                                    // :206-207
+    if (march && it > 0) {   // p = r + beta p (:223-229) folded into A p (:206-207)
+      hipLaunchKernelGGL((k_cg_ap_fold<T, RESID_TY>), dim3(mb), dim3(MARCH_THREADS), 0, op.s, r, p, p2, ap, d.L(),
+                         G, partial, sc, cur, nxt);
+      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, op.s, partial, mb, sc + 2);
+      std::swap(p, p2);
+    } else if (march) {                                             // :206-207
       hipLaunchKernelGGL((k_resid_march<T, RESID_TY, 3>), dim3(mb), dim3(MARCH_THREADS), 0, op.s, p,
                          (const float*)nullptr, (const T*)nullptr, ap, (float*)nullptr, (T*)nullptr, d.L(), G, partial);
       hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, op.s, partial, mb, sc + 2);
@@ -137,7 +145,7 @@ static void cg_iterate(Ops<T>& op, DevLevel<T>& d, T* x, const T* b, T* r, T* p,
     hipLaunchKernelGGL((k_cg_xr<T>), dim3(eb), eblk, 0, op.s, x, r, p, ap, sc, cur, d.L(), zero_x && it == 0,
                        partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, op.s, partial, eb, sc + nxt);   // :221
-    if (it + 1 < nu)
+    if (!march && it + 1 < nu)
       hipLaunchKernelGGL((k_cg_p<T>), dim3(eb), eblk, 0, op.s, p, r, sc, nxt, cur, d.L());
   }
 }
@@ -209,7 +217,8 @@ static void smooth_level(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, i
   bool fused = c->opt.fused_smoother && c->opt.smoother == 0 && nu > 0;
   if (prolong_first) op.prolong_add(d, h.lv[l + 1], d.x);
   if (c->opt.smoother == 1) {
-    cg_iterate(op, d, d.x, d.b, d.r, cg->p[l], cg->ap[l], nu, zero_start, c->d_partial, c->d_scalars + 8);
+    cg_iterate(op, d, d.x, d.b, d.r, cg->p[l], cg->p2[l], cg->ap[l], nu, zero_start, c->d_partial,
+               c->d_scalars + 8);
     return;
   }
   if (!fused) {
@@ -900,12 +909,15 @@ int pepsmg_cg_iterator(const pepsmg_grid* grid, double* sol, const double* coeff
   if (rc) return rc;
   Ops<double> op{o.s};
   double* p = nullptr;
+  double* p2 = nullptr;
   double* ap = nullptr;
   HIP_TRY(hipMalloc(&p, o.d.ncell * sizeof(double)));
+  HIP_TRY(hipMalloc(&p2, o.d.ncell * sizeof(double)));
   HIP_TRY(hipMalloc(&ap, o.d.ncell * sizeof(double)));
-  cg_iterate(op, o.d, o.d.x, o.d.b, o.d.r, p, ap, maxiteration, false, o.partial, o.sc);
+  cg_iterate(op, o.d, o.d.x, o.d.b, o.d.r, p, p2, ap, maxiteration, false, o.partial, o.sc);
   rc = o.download(sol, o.d.x, 0);
   hipFree(p);
+  hipFree(p2);
   hipFree(ap);
   return rc;
 }
