@@ -265,11 +265,11 @@ static void coarse_solve_level(pepsmg_ctx* c, Hier<float>& h, float omega, doubl
 // whose level-1 right-hand side has been restricted already
 // (src/multigrid.f90:1936-1962).  Levels from the switch level s down run
 // in the persistent single-workgroup kernel (RBGS smoother only); the
-// coarsest solve is FP64 in every precision.
+// coarsest solve is FP64 in every precision.  nu / nu_post: smoothing steps
+// on the way down / up (equal in FP64, the reference's V(nu, nu)).
 template <typename T>
 static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int nu, int nu_post, T omega,
-                               double tol,
-                               int* slot) {
+                               double tol, int* slot) {
   int L = c->nlevel;
   int s = L + 1;
   PCArgs<T> A;
