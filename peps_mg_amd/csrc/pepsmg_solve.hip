@@ -17,14 +17,15 @@
 namespace pepsmg {
 
 // ---------------------------------------------------------------------------
-// CG iterator (src/cg_poisson_matrix.f90:157-241) on device arrays, four
-// passes per iteration: Ap = A p with the partial sums of (Ap, p) (one
-// z-march pass), x += alpha p and r -= alpha Ap with the partial sums of
-// (r, r) (one elementwise pass), p = r + beta p (one pass, skipped after the
-// last iteration: p is local to the reference routine), and the final
-// reductions.  Scalars stay on the device: sc[it & 1] = (r, r) at the start
-// of iteration it (temp1), sc[2] = (Ap, p) (temp2); alpha and beta are
-// formed in FP64 by every thread as the reference does (:209, :223).
+// CG iterator (src/cg_poisson_matrix.f90:157-241) on device arrays: Ap = A p
+// with the partial sums of (Ap, p) (one z-march pass), x += alpha p and
+// r -= alpha Ap with the partial sums of (r, r) (one elementwise pass), and
+// the final reductions; the direction update p = r + beta p of an iteration
+// is folded into the next iteration's A p pass (k_cg_ap_fold; after the
+// last iteration it is not needed: p is local to the reference routine).
+// Scalars stay on the device: sc[it & 1] = (r, r) at the start of iteration
+// it (temp1), sc[2] = (Ap, p) (temp2); alpha and beta are formed in FP64 by
+// every thread as the reference does (:209, :223).
 constexpr int CG_WAVES = 4;
 
 // rows (j, k) of a level dealt to waves; each lane handles pairs (i odd, i+1)
