@@ -4,7 +4,9 @@
 !>
 !>   &gpu smoother = 0|1 (RBGS | CG), precision = 0|1 (FP64 | FP32 V-cycles in
 !>        FP64 refinement), coarse_switch, inner_cycles, max_refine, verbose,
-!>        rhs_mode = 0|1, seed, output_file, write_solution /
+!>        inner_nu, inner_nu_post, inner_omega (the FP32 V-cycles' pre- and
+!>        post-smoothing steps and SOR factor), rhs_mode = 0|1, seed,
+!>        output_file, write_solution /
 !>
 !> rhs_mode 0 is the reference's "Problem 1" cosine right-hand side with its
 !> RMS check (src/main_peps.f90:78-90, :172-182); rhs_mode 1 is the synthetic
@@ -39,6 +41,8 @@ program main_peps
     integer(kind=4) :: CGA_level, CGPSA_level(3)
     real(kind=8)    :: tolerance, omega_sor
     integer(kind=4) :: smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, rhs_mode
+    integer(kind=4) :: inner_nu, inner_nu_post
+    real(kind=8)    :: inner_omega
     integer(kind=8) :: seed
     character(len=256) :: output_file
     logical :: write_solution
@@ -60,9 +64,10 @@ program main_peps
                        CGA_level, CGPSA_level, omega_sor
     namelist /coefficients/ alpha_x, alpha_y, alpha_z
     namelist /gpu/ smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, rhs_mode, seed, output_file, &
-                   write_solution
+                   write_solution, inner_nu, inner_nu_post, inner_omega
 
     smoother = 0; precision = 0; coarse_switch = 4096; inner_cycles = 2; max_refine = 30; verbose = 0
+    inner_nu = 2; inner_nu_post = 0; inner_omega = 1.35d0
     rhs_mode = 0; seed = 20210501_8; output_file = ''; write_solution = .false.
     alpha_x = 1.0d0; alpha_y = 1.0d0; alpha_z = 1.0d0
 
@@ -111,7 +116,8 @@ program main_peps
     print '(a)', '[Poisson] Poisson matrix constructed.'
     print '(a)', '[Poisson] Start solving equations.'
 
-    call multigrid_set_options(smoother, precision, coarse_switch, inner_cycles, max_refine, verbose)
+    call multigrid_set_options(smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, &
+                               inner_nu, inner_nu_post, inner_omega)
     call system_clock(t0, rate)
     call multigrid_create(s_domain, number_of_levels, number_of_vcycles, aggregation_method, CGA_level, CGPSA_level)
     call multigrid_solve_vcycle(s_domain%x, s_domain%r, a_poisson, s_domain%b, s_domain, maxiteration, &

@@ -43,10 +43,15 @@ contains
     end subroutine multigrid_create
 
     !> Options with no counterpart in the reference (smoother, precision,
-    !> persistent coarse-level switch, refinement schedule, verbosity); they
-    !> apply to the current hierarchy and to hierarchies created later.
-    subroutine multigrid_set_options(smoother, precision, coarse_switch, inner_cycles, max_refine, verbose)
+    !> persistent coarse-level switch, refinement schedule, verbosity; the
+    !> FP32 inner V-cycles' pre- and post-smoothing steps and SOR factor if
+    !> given); they apply to the current hierarchy and to hierarchies
+    !> created later.
+    subroutine multigrid_set_options(smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, &
+                                     inner_nu, inner_nu_post, inner_omega)
         integer(kind=4), intent(in) :: smoother, precision, coarse_switch, inner_cycles, max_refine, verbose
+        integer(kind=4), intent(in), optional :: inner_nu, inner_nu_post
+        real(kind=8), intent(in), optional :: inner_omega
         if (.not. mg_opt_set) call pepsmg_default_options(mg_opt)
         mg_opt%smoother = smoother
         mg_opt%precision = precision
@@ -54,6 +59,9 @@ contains
         mg_opt%inner_cycles = inner_cycles
         mg_opt%max_refine = max_refine
         mg_opt%verbose = verbose
+        if (present(inner_nu)) mg_opt%inner_nu = inner_nu
+        if (present(inner_nu_post)) mg_opt%inner_nu_post = inner_nu_post
+        if (present(inner_omega)) mg_opt%inner_omega = inner_omega
         mg_opt_set = .true.
         if (c_associated(mg_ctx)) call pepsmg_check(pepsmg_set_options(mg_ctx, mg_opt), 'multigrid_set_options')
     end subroutine multigrid_set_options

@@ -114,6 +114,35 @@ def test_driver_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_mixed_precision_inner_schedule(tmp_path):
+    """precision = 1 with an asymmetric inner V-cycle set through the &gpu
+    namelist: the driver's solution meets the tolerance against an
+    independent FP64 residual and stays within 1e-8 of the oracle's FP64
+    solution."""
+    import refmath as RM
+    from oracle import oracle as O
+    from peps_mg_amd import _lib
+    exe = ensure_built()
+    n, nl = 32, 3
+    write_input(tmp_path, n=n, nl=nl, gpu=dict(rhs_mode=1, seed=5, output_file="'x.bin'", precision=1,
+                                               inner_nu=2, inner_nu_post=1, inner_omega="1.2d0"))
+    p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    x = np.fromfile(tmp_path / "x.bin", dtype=np.float64).reshape(n + 2, n + 2, n + 2)
+    g = O.Grid(n, n, n)
+    b = O.padded(n, n, n)
+    _lib.check(_lib.lib().pepsmg_fill_rhs_host(n, n, n, _lib.dptr(b), 5, -1.0, -3 * np.pi ** 2, _lib.dptr(g.xg),
+                                               _lib.dptr(g.yg), _lib.dptr(g.zg)))
+    K = RM.laplacian(g.dxm[1:-1], g.dym[1:-1], g.dzm[1:-1])
+    r = RM.flat(b) - K @ RM.flat(x)
+    assert np.linalg.norm(r) / np.linalg.norm(RM.flat(b)) < 1.01e-10
+    mo = O.Multigrid(g, nl, 60)
+    xo, ro = O.padded(n, n, n), O.padded(n, n, n)
+    mo.solve_vcycle(xo, ro, g.matrix(), b, 2, 1e-10, 1.0)
+    assert np.abs(x - xo).max() <= 1e-8 * np.abs(xo).max()
+
+
+@pytest.mark.gpu
 def test_driver_cosine_problem(tmp_path):
     """The reference's Problem 1 (cosine RHS over -l/2 <= x,y,z <= l/2)
     converges with O(h^2) RMS error."""
