@@ -17,8 +17,15 @@
 
 namespace pepsmg {
 
-// tile height of the fine-level residual kernels (norm, refinement update)
+// tile height of the fine-level residual kernels (norm, refinement update,
+// CG steps)
 constexpr int RESID_TY = 16;
+// the refinement kernels' tile height on levels of 512 rows or more: 32
+// rows halve the share of the one-row y ring re-read per tile (512^3 mixed
+// solve 18.75 against 19.1 ms; two 256-thread blocks per CU by LDS either
+// way).  Smaller levels keep 16: at about 2048 blocks, 32-row tiles leave
+// z-chunks of 4 planes at 256^3, half of them halo.
+constexpr int RESID_TY_BIG = 32;
 
 // ---------------------------------------------------------------------------
 // Launch helpers

@@ -431,7 +431,8 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   }
   DevLevel<double>& f = h.lv[0];
   DevLevel<float>& f32 = g.lv[0];
-  MarchGrid G = Ops<double>::march_grid(f, RESID_TY, false);
+  const bool big = f.ny >= 512;   // refinement kernels' tile height RESID_TY_BIG, else RESID_TY
+  MarchGrid G = Ops<double>::march_grid(f, big ? RESID_TY_BIG : RESID_TY, false);
   int nb = G.gx * G.gy * G.nch;
   // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x, x2
   // the caller's padded layout (pitches nx+2, (nx+2)(ny+2); offset of cell (1,1,1))
@@ -443,8 +444,12 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   // r32 = b - A x and its norm, x and b read from the caller's padded
   // arrays and stored in the solver's layout (x into x2, then swap)
   auto start_from_caller = [&]() {
-    hipLaunchKernelGGL((k_refine_start<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, ux + uo,
-                       ub + uo, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
+    if (big)
+      hipLaunchKernelGGL((k_refine_start<RESID_TY_BIG>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream,
+                         ux + uo, ub + uo, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
+    else
+      hipLaunchKernelGGL((k_refine_start<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, ux + uo,
+                         ub + uo, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
     std::swap(f.x, f.x2);
   };
@@ -452,9 +457,15 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   // x2.  last: x + e goes only into the caller's array, with its norm; if
   // the step then misses the tolerance, start_from_caller restores x and r32
   auto refine_update = [&](bool last) {
-    if (last)
+    if (last && big)
+      hipLaunchKernelGGL((k_refine_last<RESID_TY_BIG>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream,
+                         f.x, f32.x, f.b, (float*)nullptr, (double*)nullptr, f.L(), U, G, c->d_partial, ux + uo);
+    else if (last)
       hipLaunchKernelGGL((k_refine_last<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
                          f32.x, f.b, (float*)nullptr, (double*)nullptr, f.L(), U, G, c->d_partial, ux + uo);
+    else if (big)
+      hipLaunchKernelGGL((k_resid_march<double, RESID_TY_BIG, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0,
+                         c->stream, f.x, f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
     else
       hipLaunchKernelGGL((k_resid_march<double, RESID_TY, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0,
                          c->stream, f.x, f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
