@@ -386,8 +386,8 @@ def cpu_baseline(n, lz, fp64_cycles):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not implemented", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "the solver runs on one GPU; sharding a grid over several GPUs is not "
+                                   "implemented", "n_gpus": a.gpus}))
         sys.exit(2)
     if a.kernels_only:
         kernels_only(a)

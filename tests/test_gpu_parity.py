@@ -172,24 +172,20 @@ def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty, inner):
     assert err < 1e-8, err
 
 
-def test_mixed_refinement_end_states():
-    """Mixed precision over a range of tolerances.  The refinement step
-    predicted to be the last stores x + e only into sol; when it misses the
-    tolerance, x and the FP32 residual are restored from sol and the solve
-    goes on.  Across the sweep both outcomes occur; in every case the
-    returned sol, rsd (host and device entry points, rsd aliasing rhs on the
-    device) and rho agree with an independent FP64 evaluation of b - A x
-    (scipy sparse Laplacian)."""
+def _refinement_sweep(dims, lens, nlevel, tols):
+    """Mixed-precision solves of one grid over the tolerances `tols`; checks
+    sol, rsd and rho of each against an independent FP64 evaluation of
+    b - A x (scipy sparse Laplacian) and returns the set of refinement step
+    counts seen."""
     import torch
-    n = 48
-    g = O.Grid(n, n, n)
-    b = -O.uniform_field(n, n, n, 5)
+    g = O.Grid(*dims, l=lens)
+    b = -O.uniform_field(*dims, 5)
     K = RM.laplacian(g.dxm[1:-1], g.dym[1:-1], g.dzm[1:-1])
-    mg = Multigrid(sdm_from(g), 3, 80, precision=1)
+    mg = Multigrid(sdm_from(g), nlevel, 80, precision=1)
     bmax = np.abs(b).max()
     steps = set()
-    for tol in np.geomspace(1e-6, 3e-13, 15):
-        xg, rg = O.padded(n, n, n), O.padded(n, n, n)
+    for tol in tols:
+        xg, rg = O.padded(*dims), O.padded(*dims)
         res = mg.solve_vcycle(xg, rg, None, b, 2, tol, 1.0)
         assert res.converged and res.rho < tol
         r = RM.flat(b) - K @ RM.flat(xg)
@@ -203,7 +199,26 @@ def test_mixed_refinement_end_states():
         torch.cuda.synchronize()
         assert res_d.refine_steps == res.refine_steps and res_d.rho == res.rho
         assert np.array_equal(d_x.cpu().numpy(), xg) and np.array_equal(d_b.cpu().numpy(), rg)
+    mg.destroy()
+    return steps
+
+
+def test_mixed_refinement_end_states():
+    """Mixed precision over a range of tolerances.  The refinement step
+    predicted to be the last stores x + e only into sol; when it misses the
+    tolerance, x and the FP32 residual are restored from sol and the solve
+    goes on.  Across the sweep both outcomes occur; in every case the
+    returned sol, rsd (host and device entry points, rsd aliasing rhs on the
+    device) and rho agree with an independent FP64 evaluation of b - A x."""
+    steps = _refinement_sweep((48, 48, 48), (1, 1, 1), 3, np.geomspace(1e-6, 3e-13, 15))
     assert len(steps) >= 3, steps   # the sweep spans several step counts
+
+
+def test_mixed_refinement_tall_level():
+    """The same checks on a level of 512 rows, where the refinement kernels
+    march 32-row tiles instead of 16."""
+    steps = _refinement_sweep((64, 512, 32), (0.125, 1, 0.0625), 3, np.geomspace(1e-5, 1e-12, 5))
+    assert len(steps) >= 2, steps
 
 
 @pytest.mark.parametrize("precision,where", [(0, (0, 3, 3)), (1, (0, 3, 3)), (1, (3, 9, 3)), (1, (3, 3, 9)),
