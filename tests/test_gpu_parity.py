@@ -214,10 +214,13 @@ def test_mixed_refinement_end_states():
     assert len(steps) >= 3, steps   # the sweep spans several step counts
 
 
-def test_mixed_refinement_tall_level():
-    """The same checks on a level of 512 rows, where the refinement kernels
-    march 32-row tiles instead of 16."""
-    steps = _refinement_sweep((64, 512, 32), (0.125, 1, 0.0625), 3, np.geomspace(1e-5, 1e-12, 5))
+@pytest.mark.parametrize("dims", [(64, 512, 32), (80, 528, 24)], ids=["whole-tiles", "partial-tiles"])
+def test_mixed_refinement_tall_level(dims):
+    """The same checks on levels of 512 rows or more, where the refinement
+    kernels march 32-row tiles instead of 16 (80 x 528: the last column and
+    row tiles are partial)."""
+    nx, ny, nz = dims
+    steps = _refinement_sweep(dims, (nx / 512, ny / 512, nz / 512), 3, np.geomspace(1e-5, 1e-12, 5))
     assert len(steps) >= 2, steps
 
 
