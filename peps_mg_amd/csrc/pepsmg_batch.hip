@@ -726,12 +726,8 @@ static int batch_build(pepsmg_batch* B, int nu, double tol, double omega) {
   for (int p = 0; p < 2; p++)
     if (!B->sched[p].host.empty())
       HIP_TRY(hipMemcpy(B->d_arena + offs[p], B->sched[p].host.data(), B->sched[p].host.size(), hipMemcpyHostToDevice));
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void*)kb_coarse<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)B->lds_max);
-    hipFuncSetAttribute((const void*)kb_coarse<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)B->lds_max);
-    attr = true;
-  }
+  HIP_TRY(set_dynamic_lds((const void*)kb_coarse<double>, B->lds_max));
+  HIP_TRY(set_dynamic_lds((const void*)kb_coarse<float>, B->lds_max));
   for (int p = 0; p < (mixed ? 2 : 1); p++) {
     HIP_TRY(hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal));
     const char* base = B->d_arena + offs[p];
@@ -773,6 +769,7 @@ int pepsmg_batch_solve_device(pepsmg_batch* B, double* const* d_sol, const doubl
     int rc = batch_build(B, maxiteration, tolerance, omega_sor);
     if (rc) return rc;
   }
+  (void)hipGetLastError();   // an error the caller's earlier HIP calls left is not this solve's
   HIP_TRY(hipEventRecord(B->ev0, st));
   // pack the caller's padded arrays; reject non-zero ghosts
   HIP_TRY(hipMemsetAsync(B->d_flag, 0, sizeof(int), st));
@@ -843,6 +840,7 @@ int pepsmg_batch_solve_device(pepsmg_batch* B, double* const* d_sol, const doubl
   }
   HIP_TRY(hipEventRecord(B->ev1, st));
   HIP_TRY(hipEventSynchronize(B->ev1));
+  HIP_TRY(hipGetLastError());   // a launch the runtime rejected leaves its error here
   float ms = 0;
   hipEventElapsedTime(&ms, B->ev0, B->ev1);
   if (mixed) {

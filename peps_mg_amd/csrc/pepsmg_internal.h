@@ -3,7 +3,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <set>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pepsmg.h"
@@ -22,6 +25,24 @@ long pitch_for(int nx);
     hipError_t e_ = (expr);                         \
     if (e_ != hipSuccess) return hip_fail(e_, #expr); \
   } while (0)
+
+// Raises kernel kern's dynamic LDS limit to `bytes` on the current device.
+// The attribute is per device, so it is set once per (kernel, device, size);
+// a failure is returned and also stays pending for hipGetLastError, which
+// the solve entry points check after their work has completed.
+inline hipError_t set_dynamic_lds(const void* kern, size_t bytes) {
+  static std::mutex mu;
+  static std::set<std::tuple<const void*, int, size_t>> done;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  auto key = std::make_tuple(kern, dev, bytes);
+  if (done.count(key)) return hipSuccess;
+  e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) done.insert(key);
+  return e;
+}
 
 // Host copy of one level's metrics and transfer weights (index 0..n+1).
 struct HostLevel {

@@ -238,12 +238,7 @@ static void coarse_solve64(pepsmg_ctx* c, DevLevel<double>& d, double omega, dou
 
 template <typename T>
 static void pc_launch(pepsmg_ctx* c, const PCArgs<T>& A, size_t lds) {
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void*)k_coarse_vcycle<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)c->lds_max);
-    attr = true;
-  }
+  set_dynamic_lds((const void*)k_coarse_vcycle<T>, c->lds_max);   // a failure surfaces at the solve's end
   hipLaunchKernelGGL((k_coarse_vcycle<T>), dim3(1), dim3(PC_THREADS), lds, c->stream, A);
 }
 
@@ -611,6 +606,7 @@ static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_sta
   std::vector<int> cit;
   double res0 = 0, rho = 0;
   int ncyc = 0, nref = 0, rc;
+  (void)hipGetLastError();   // an error the caller's earlier HIP calls left is not this solve's
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   if (c->opt.precision == 1)
     rc = solve_mixed(c, nu, tol, omega, hist, cit, &res0, &ncyc, &nref, &rho, ux, ub);
@@ -619,6 +615,7 @@ static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_sta
   if (rc) return rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev1));
+  HIP_TRY(hipGetLastError());   // a launch the runtime rejected leaves its error here
   if (st) {
     float ms = 0;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
