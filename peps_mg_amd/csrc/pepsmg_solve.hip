@@ -805,6 +805,7 @@ struct OneLevel {
       h.dm[a].assign(dms[a], dms[a] + h.n[a] + 2);
       h.dg[a].assign(dgs[a], dgs[a] + h.n[a] + 2);
     }
+    (void)hipGetLastError();   // an error the caller's earlier HIP calls left is not this call's
     h.setup_coeffs();
     d.nx = h.n[0]; d.ny = h.n[1]; d.nz = h.n[2];
     d.px = pitch_for(d.nx);
@@ -839,6 +840,7 @@ struct OneLevel {
     HIP_TRY(hipMemcpy(pad_b, host, np * sizeof(double), hipMemcpyHostToDevice));
     unpack(s, pad_b, src, d, zero_ghosts);
     HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipGetLastError());   // the result is the last step: report a launch the runtime rejected
     HIP_TRY(hipMemcpy(host, pad_b, np * sizeof(double), hipMemcpyDeviceToHost));
     return PEPSMG_OK;
   }
@@ -1005,7 +1007,10 @@ int pepsmg_vv_dot(int32_t nx, int32_t ny, int32_t nz, const double* x, const dou
   if (rc) return rc;
   Ops<double> op{o.s};
   op.dot(o.d, o.d.x, o.d.b, o.partial, o.sc);
-  return o.scalar(0, result);
+  rc = o.scalar(0, result);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());   // a launch the runtime rejected leaves its error here
+  return PEPSMG_OK;
 }
 
 }  // extern "C"
