@@ -39,7 +39,8 @@ enum {
   PEPSMG_ERR_MATRIX = -4,     /* coeff differs from the metric rediscretisation */
   PEPSMG_ERR_HIP = -5,        /* HIP runtime error                            */
   PEPSMG_ERR_NOMEM = -6,      /* device allocation failed                     */
-  PEPSMG_ERR_NOTCONV = -7     /* solver stopped at its cap above tolerance (solver entry points only) */
+  PEPSMG_ERR_NOTCONV = -7,    /* solver stopped at its cap above tolerance (solver entry points only) */
+  PEPSMG_ERR_DIVERGED = -8    /* a V-cycle solve's residual norm stopped being finite */
 };
 
 /* One subdomain's metrics (type subdomain, src/geometry.f90:59-118): cell

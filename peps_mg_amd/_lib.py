@@ -15,7 +15,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 OK = 0
-ERR_ARG, ERR_LEVELS, ERR_GHOST, ERR_MATRIX, ERR_HIP, ERR_NOMEM, ERR_NOTCONV = -1, -2, -3, -4, -5, -6, -7
+ERR_ARG, ERR_LEVELS, ERR_GHOST, ERR_MATRIX, ERR_HIP, ERR_NOMEM, ERR_NOTCONV, ERR_DIVERGED = -1, -2, -3, -4, -5, -6, -7, -8
 
 
 class PepsmgError(RuntimeError):
@@ -119,11 +119,40 @@ def check(rc):
     return rc
 
 
-def dptr(a):
+def dptr(a, name="array"):
+    """Pointer to a C-contiguous float64 numpy array (None passes through).
+    Raises PepsmgError(ERR_ARG) for anything else: the C entry points read
+    and write whole padded grids through the pointer."""
     if a is None:
         return None
-    assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous, "need C-contiguous float64"
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise PepsmgError(ERR_ARG, f"{name} must be a C-contiguous float64 numpy array")
     return a.ctypes.data_as(_dp)
+
+
+def padded_ptr(a, nx, ny, nz, name):
+    """Pointer to a padded grid array a(0:nx+1,0:ny+1,0:nz+1), numpy shape
+    (nz+2, ny+2, nx+2); a wrong shape raises PepsmgError(ERR_ARG) (the C
+    entry points copy (nx+2)(ny+2)(nz+2) values in and out)."""
+    if a is None:
+        raise PepsmgError(ERR_ARG, f"{name} is None")
+    want = (nz + 2, ny + 2, nx + 2)
+    if getattr(a, "shape", None) != want:
+        raise PepsmgError(ERR_ARG, f"{name} has shape {getattr(a, 'shape', None)}, expected {want} "
+                                   f"(padded (0:nx+1,0:ny+1,0:nz+1) in C order)")
+    return dptr(a, name)
+
+
+def coeff_ptr(a_poisson, nx, ny, nz):
+    """Pointer to a matrix's AoS coefficients coeff(0:6,nx,ny,nz), numpy
+    shape (nz, ny, nx, 7), or None when no matrix is given."""
+    coeff = None if a_poisson is None else a_poisson.coeff
+    if coeff is None:
+        return None
+    want = (nz, ny, nx, 7)
+    if getattr(coeff, "shape", None) != want:
+        raise PepsmgError(ERR_ARG, f"coeff has shape {getattr(coeff, 'shape', None)}, expected {want}")
+    return dptr(coeff, "coeff")
 
 
 def make_grid(nx, ny, nz, dxm, dxg, dym, dyg, dzm, dzg):

@@ -150,8 +150,10 @@ kb_dot_final(const double* __restrict__ partial, const int* __restrict__ poff, c
 }
 
 // After a step: rho_i = sqrt(out_i / res0_i) into the item's history; items
-// that reached rho < tol or their step cap are switched off.  init: the
-// first call records res0 (an item with res0 = 0 is already solved).
+// that reached rho < tol or their step cap, or whose rho is no longer
+// finite, are switched off.  init: the first call records res0 (an item
+// with res0 = 0 is already solved, one with a non-finite res0 has a NaN or
+// infinity in its input; neither runs).
 // nact receives the number of items still active.
 __global__ void __launch_bounds__(256)
 kb_check(const double* __restrict__ out, double* __restrict__ res0, double* __restrict__ hist, int hcap,
@@ -172,7 +174,7 @@ kb_check(const double* __restrict__ out, double* __restrict__ res0, double* __re
         int s = steps[i];
         if (s < hcap) hist[(long)i * hcap + s] = rho;
         steps[i] = s + 1;
-        if (rho < tol || s + 1 >= maxsteps) active[i] = 0;
+        if (rho < tol || s + 1 >= maxsteps || !isfinite(rho)) active[i] = 0;
       }
     }
     if (active[i]) atomicAdd(&cnt, 1);
@@ -848,10 +850,16 @@ int pepsmg_batch_solve_device(pepsmg_batch* B, double* const* d_sol, const doubl
     // recorded schedules (made from the pre-solve state) stay valid
     for (auto& it : B->items) std::swap(it.h64.lv[0].x, it.h64.lv[0].x2);
   }
+  std::vector<double> res0(n), hist((size_t)n * B->hcap);
+  HIP_TRY(hipMemcpy(res0.data(), B->d_res0, n * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(hist.data(), B->d_hist, hist.size() * sizeof(double), hipMemcpyDeviceToHost));
+  int bad_input = -1, bad_rho = -1;
+  for (int i = 0; i < n; i++) {
+    int k = steps[i];
+    if (!std::isfinite(res0[i]) && bad_input < 0) bad_input = i;
+    if (k > 0 && !std::isfinite(hist[(size_t)i * B->hcap + std::min(k, B->hcap) - 1]) && bad_rho < 0) bad_rho = i;
+  }
   if (stats) {
-    std::vector<double> res0(n), hist((size_t)n * B->hcap);
-    HIP_TRY(hipMemcpy(res0.data(), B->d_res0, n * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hist.data(), B->d_hist, hist.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) {
       pepsmg_stats& s = stats[i];
       int k = steps[i];
@@ -866,6 +874,10 @@ int pepsmg_batch_solve_device(pepsmg_batch* B, double* const* d_sol, const doubl
       s.device_ms = ms;
     }
   }
+  if (bad_input >= 0)
+    return fail(PEPSMG_ERR_ARG, "item %d: |b - A x0|^2 = %g is not finite (rhs or sol holds NaN or Inf)", bad_input,
+                res0[bad_input]);
+  if (bad_rho >= 0) return fail(PEPSMG_ERR_DIVERGED, "item %d: the relative residual is no longer finite", bad_rho);
   return PEPSMG_OK;
 }
 

@@ -297,7 +297,24 @@ static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>*
 
 static void print_cycle(int cyc, double rv, double r0) {
   std::printf("[MG] cycle = %4d, Error: %15.7E %15.7E %15.7E\n", cyc, std::sqrt(rv), std::sqrt(r0),
-              std::sqrt(rv / r0));
+              r0 > 0.0 ? std::sqrt(rv / r0) : 0.0);
+}
+
+// The initial residual norm |b - A x0|^2.  Not finite: rhs or sol holds a
+// NaN or an infinity (the reference would cycle on NaN until its cap).
+// Zero: x0 solves the system exactly (the reference would form sqrt(0/0)
+// and run every cycle); the solve then stops with rho = 0.
+static int check_res0(double res0) {
+  if (!std::isfinite(res0))
+    return fail(PEPSMG_ERR_ARG, "|b - A x0|^2 = %g is not finite: rhs or sol holds NaN or Inf", res0);
+  return PEPSMG_OK;
+}
+
+// rho of a cycle's residual |b - A x|^2 = r2 against res0 (0 when res0 is 0)
+static double rel_residual(double r2, double res0) { return res0 > 0.0 ? std::sqrt(r2 / res0) : 0.0; }
+
+static int diverged(int cyc, double rho) {
+  return fail(PEPSMG_ERR_DIVERGED, "the relative residual became %g after %d cycle(s)", rho, cyc);
 }
 
 // FP64 V-cycles (src/multigrid.f90:1907-1976) on level-0 arrays already in
@@ -326,14 +343,21 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
   const bool fold = c->opt.fused_smoother && c->opt.smoother == 0 && nu > 0 && c->ncycle > 0;
   const int nbn = fold ? op.fused_blocks(f) : 0;
   double res0 = 0.0;
+  int ncyc = 0;
+  double rho = 1.0;
+  int first = 1;   // first cycle to run
   if (!fold) {
     op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 0);  // :1924-1925
     HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     res0 = hs[0];
+    int rc = check_res0(res0);
+    if (rc) return rc;
+    if (res0 == 0.0) {   // x0 is exact: no cycle
+      rho = 0.0;
+      first = c->ncycle + 1;
+    }
   }
-  int ncyc = 0;
-  double rho = 1.0;
   // the per-cycle factor of the last two cycles predicts that the next
   // check meets the tolerance (factor-2 margin)
   auto likely_converged = [&]() {
@@ -341,14 +365,19 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
     if (n < 2 || !(hist[n - 2] > 0.0)) return false;
     return hist[n - 1] * (hist[n - 1] / hist[n - 2]) < 2.0 * tol;
   };
-  auto check = [&](int cyc, double r2) {   // :1971-1973
-    rho = std::sqrt(r2 / res0);
+  // :1971-1973; 1: rho < tol (stop), 0: go on, < 0: error (a non-finite
+  // initial norm, or a cycle whose norm is no longer finite)
+  auto check = [&](int cyc, double r2) -> int {
+    int rc = check_res0(res0);
+    if (rc) return rc;
+    rho = rel_residual(r2, res0);
     hist.push_back(rho);
     ncyc = cyc;
     if (c->opt.verbose) print_cycle(cyc, r2, res0);
-    return rho < tol;
+    if (!std::isfinite(rho)) return diverged(cyc, rho);
+    return rho < tol ? 1 : 0;
   };
-  for (int cyc = 1; cyc <= c->ncycle; cyc++) {
+  for (int cyc = first; cyc <= c->ncycle; cyc++) {
     if (fold) {
       // first pre-smoothing iteration, with |b - A x|^2 of the incoming x
       op.fused_iter(f, omega, 2, c->d_partial);
@@ -366,7 +395,9 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
         if (ahead) op.fused_iter_into(f, omega, f.r);
         HIP_TRY(hipEventSynchronize(c->ev_norm));
         res0 = c->h_scalars[0];
-        if (check(cyc - 1, c->h_scalars[1])) {
+        int stop = check(cyc - 1, c->h_scalars[1]);
+        if (stop < 0) return stop;
+        if (stop) {
           std::swap(f.x, f.x2);   // the solution is the iteration's input
           break;
         }
@@ -388,7 +419,9 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
       HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
       if (fold) res0 = hs[0];
-      if (check(cyc, hs[1])) break;
+      int stop = check(cyc, hs[1]);
+      if (stop < 0) return stop;
+      if (stop) break;
     }
   }
   *res0_out = res0;
@@ -498,15 +531,17 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   if (rc) return rc;
   double res0 = c->h_scalars[0];
   *res0_out = res0;
+  rc = check_res0(res0);
+  if (rc) return rc;
   int ncyc = 0, nref = 0;
-  double rho = 1.0;
+  double rho = res0 == 0.0 ? 0.0 : 1.0;   // x0 exact: no refinement step
   // Per-V-cycle reduction factor of the previous step.  A step that is
   // predicted (with a factor-2 margin) to reach the tolerance in fewer than
   // inner_cycles cycles runs only those: typically the last step, which needs
   // a small reduction, while FP32 storage caps every step at about 1e-3 (so
   // a capped step overstates the factor and the prediction stays safe).
   double fcyc = 0.0;
-  for (int s = 1; s <= c->opt.max_refine && ncyc < c->ncycle; s++) {
+  for (int s = 1; s <= c->opt.max_refine && ncyc < c->ncycle && res0 > 0.0; s++) {
     int nin = c->opt.inner_cycles;
     if (fcyc > 0.0 && fcyc < 1.0) {
       double need = 0.5 * tol / rho, red = fcyc;
@@ -517,6 +552,7 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
       }
       nin = k;
     }
+    nin = std::min(nin, c->ncycle - ncyc);   // number_of_vcycles caps the FP32 V-cycles of the solve
     double rho_prev = rho;
     for (int q = 0; q < nin; q++) {
       int* slot = (ncyc < c->citers_cap) ? c->d_citers + ncyc : nullptr;
@@ -534,10 +570,17 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     rc = read_norm(!stop_likely && s < c->opt.max_refine && ncyc < c->ncycle);
     if (rc) return rc;
     const double r2 = c->h_scalars[0];
-    rho = std::sqrt(r2 / res0);
+    rho = rel_residual(r2, res0);
     hist.push_back(rho);
     nref = s;
     if (c->opt.verbose) print_cycle(s, r2, res0);
+    if (!std::isfinite(rho)) {
+      // (a step predicted to be the last has written its x + e into sol)
+      *ncyc_out = ncyc;
+      *nref_out = nref;
+      *rho_out = rho;
+      return diverged(ncyc, rho);
+    }
     if (rho < tol) {
       c->sol_out_done = last;
       break;
@@ -612,7 +655,8 @@ static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_sta
     rc = solve_mixed(c, nu, tol, omega, hist, cit, &res0, &ncyc, &nref, &rho, ux, ub);
   else
     rc = solve_fp64(c, nu, tol, omega, st, hist, cit, &res0, &ncyc, &rho);
-  if (rc) return rc;
+  if (rc && rc != PEPSMG_ERR_DIVERGED) return rc;   // (a diverged solve still reports its statistics)
+  const int solve_rc = rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev1));
   HIP_TRY(hipGetLastError());   // a launch the runtime rejected leaves its error here
@@ -630,7 +674,7 @@ static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_sta
     if (st->coarse_iters)
       for (int q = 0; q < (int)cit.size() && q < st->history_cap; q++) st->coarse_iters[q] = cit[q];
   }
-  return PEPSMG_OK;
+  return solve_rc;
 }
 
 }  // namespace pepsmg
@@ -675,7 +719,10 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
   hipError_t e = hipMemcpyAsync(h_flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e != hipSuccess) rc = hip_fail(e, "pack");
   if (!rc) rc = run_solve(c, maxiteration, tolerance, omega_sor, st, direct ? d_sol : nullptr, direct ? d_rhs : nullptr);
-  if (!rc) {
+  if (rc) {
+    // nothing of this solve may still run once the caller has the error
+    (void)hipStreamSynchronize(c->stream);
+  } else {
     e = hipStreamSynchronize(c->stream);
     flag = *h_flag;
     if (e != hipSuccess) rc = hip_fail(e, "pack");
@@ -851,6 +898,26 @@ struct OneLevel {
   }
 };
 
+// A device array of doubles freed when it goes out of scope, so every exit
+// path of an entry point releases it.
+struct DevBuf {
+  double* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+  int alloc(size_t n) {
+    hipError_t e = hipMalloc(&p, n * sizeof(double));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(PEPSMG_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(double), hipGetErrorString(e));
+    }
+    return PEPSMG_OK;
+  }
+};
+
 // rbgs_solver on one level (src/rbgs_poisson_matrix.f90:73-171): kernel
 // loop with the norm reduced on the device and tested on the host.
 static int rbgs_solver_level(OneLevel& o, int maxit, double tol, double omega, int* iters, double* rel) {
@@ -919,18 +986,10 @@ int pepsmg_cg_iterator(const pepsmg_grid* grid, double* sol, const double* coeff
   if (!rc) rc = o.upload(o.d.b, rhs);
   if (rc) return rc;
   Ops<double> op{o.s};
-  double* p = nullptr;
-  double* p2 = nullptr;
-  double* ap = nullptr;
-  HIP_TRY(hipMalloc(&p, o.d.ncell * sizeof(double)));
-  HIP_TRY(hipMalloc(&p2, o.d.ncell * sizeof(double)));
-  HIP_TRY(hipMalloc(&ap, o.d.ncell * sizeof(double)));
-  cg_iterate(op, o.d, o.d.x, o.d.b, o.d.r, p, p2, ap, maxiteration, false, o.partial, o.sc);
-  rc = o.download(sol, o.d.x, 0);
-  hipFree(p);
-  hipFree(p2);
-  hipFree(ap);
-  return rc;
+  DevBuf p, p2, ap;
+  if ((rc = p.alloc(o.d.ncell)) || (rc = p2.alloc(o.d.ncell)) || (rc = ap.alloc(o.d.ncell))) return rc;
+  cg_iterate(op, o.d, o.d.x, o.d.b, o.d.r, p.p, p2.p, ap.p, maxiteration, false, o.partial, o.sc);
+  return o.download(sol, o.d.x, 0);
 }
 
 int pepsmg_cg_solver(const pepsmg_grid* grid, double* sol, const double* coeff, const double* rhs,
@@ -943,9 +1002,10 @@ int pepsmg_cg_solver(const pepsmg_grid* grid, double* sol, const double* coeff, 
   if (!rc) rc = o.upload(o.d.b, rhs);
   if (rc) return rc;
   Ops<double> op{o.s};
-  double *p = nullptr, *ap = nullptr;
-  HIP_TRY(hipMalloc(&p, o.d.ncell * sizeof(double)));
-  HIP_TRY(hipMalloc(&ap, o.d.ncell * sizeof(double)));
+  DevBuf pb, apb;
+  if ((rc = pb.alloc(o.d.ncell)) || (rc = apb.alloc(o.d.ncell))) return rc;
+  double* p = pb.p;
+  double* ap = apb.p;
   double* sc = o.sc;
   const long nrow = (long)o.d.ny * o.d.nz;
   const int eb = (int)std::max(1L, std::min<long>((nrow + CG_WAVES - 1) / CG_WAVES, RED_BLOCKS));
@@ -970,8 +1030,6 @@ int pepsmg_cg_solver(const pepsmg_grid* grid, double* sol, const double* coeff, 
     if (std::sqrt(t2 / r0) < tolerance) break;                     // :126
     hipLaunchKernelGGL((k_cg_p<double>), dim3(eb), eblk, 0, o.s, p, o.d.r, sc, nxt, cur, o.d.L());
   }
-  hipFree(p);
-  hipFree(ap);
   if (rc) return rc;
   if (iters) *iters = it;
   if (rel) *rel = std::sqrt(t2 / r0);

@@ -29,6 +29,9 @@ contains
         integer(c_int32_t) :: it
         real(c_double) :: rel
         g = pepsmg_grid_of(dm)
+        call pepsmg_check_padded(sol, dm%nx, dm%ny, dm%nz, 'sol', 'cg_solver_poisson_matrix')
+        call pepsmg_check_padded(rhs, dm%nx, dm%ny, dm%nz, 'rhs', 'cg_solver_poisson_matrix')
+        call pepsmg_check_coeff(a_poisson, dm%nx, dm%ny, dm%nz, 'cg_solver_poisson_matrix')
         call pepsmg_check(pepsmg_cg_solver(g, pepsmg_ptr3(sol), pepsmg_coeff_ptr(a_poisson), pepsmg_ptr3(rhs), &
                           maxiteration, tolerance, it, rel), 'cg_solver_poisson_matrix')
         if (myrank == 0) print '(a,i5,a,e15.7)', '[CG] Finished with total iteration = ', it, ', r_mse = ', rel
@@ -46,6 +49,9 @@ contains
         logical, intent(in)                            :: is_aggregated(0:2)
         type(pepsmg_grid_t) :: g
         g = pepsmg_grid_of(dm)
+        call pepsmg_check_padded(sol, dm%nx, dm%ny, dm%nz, 'sol', 'cg_iterator_poisson_matrix')
+        call pepsmg_check_padded(rhs, dm%nx, dm%ny, dm%nz, 'rhs', 'cg_iterator_poisson_matrix')
+        call pepsmg_check_coeff(a_poisson, dm%nx, dm%ny, dm%nz, 'cg_iterator_poisson_matrix')
         call pepsmg_check(pepsmg_cg_iterator(g, pepsmg_ptr3(sol), pepsmg_coeff_ptr(a_poisson), pepsmg_ptr3(rhs), &
                           maxiteration), 'cg_iterator_poisson_matrix')
     end subroutine cg_iterator_poisson_matrix

@@ -6,19 +6,22 @@
 !>        FP64 refinement), coarse_switch, inner_cycles, max_refine, verbose,
 !>        inner_nu, inner_nu_post, inner_omega (the FP32 V-cycles' pre- and
 !>        post-smoothing steps and SOR factor), rhs_mode = 0|1, seed,
-!>        output_file, write_solution /
+!>        output_file, rhs_file, write_solution /
 !>
 !> rhs_mode 0 is the reference's "Problem 1" cosine right-hand side with its
 !> RMS check (src/main_peps.f90:78-90, :172-182); rhs_mode 1 is the synthetic
 !> workload b = -(U[-1,1] + 3 pi^2 sin(pi x) sin(pi y) sin(pi z)) with U the
 !> SplitMix64 field of the given seed.  output_file, if set, receives the
-!> padded solution array as raw float64 (stream access).  write_solution
-!> (default .false.) writes the reference's solution files
-!> (file_write, src/main_peps.f90:204-446, on one process): solution_all.dat
-!> (x, y, z, u per interior cell, format (3D20.10,1E30.20)), solution_all.bin
-!> (one unformatted record of u(1:nx,1:ny,1:nz)) and
-!> solution_all_MPI_IO.bin (the same values as raw float64), then checks
-!> that the two binary files agree (binary_file_check, :448-484).
+!> padded solution array as raw float64 (stream access), rhs_file the padded
+!> right-hand side the same way.  write_solution (default .false.) writes the
+!> reference's seven solution files (file_write, src/main_peps.f90:204-446,
+!> on one process): solution_all.dat, solution_all_gatherv.dat and
+!> solution_all_gatherv_halo.dat (x, y, z, u per interior cell, format
+!> (3D20.10,1E30.20)), solution_all.bin, solution_all_gatherv.bin and
+!> solution_all_gatherv_halo.bin (one unformatted record of
+!> u(1:nx,1:ny,1:nz)) and solution_all_MPI_IO.bin (the same values as raw
+!> float64), then checks that solution_all.bin and the MPI-IO file agree
+!> (binary_file_check, :448-484).
 program main_peps
 
     use, intrinsic :: iso_c_binding
@@ -44,7 +47,7 @@ program main_peps
     integer(kind=4) :: inner_nu, inner_nu_post
     real(kind=8)    :: inner_omega
     integer(kind=8) :: seed
-    character(len=256) :: output_file
+    character(len=256) :: output_file, rhs_file
     logical :: write_solution
 
     type(domain), target     :: g_domain
@@ -64,11 +67,11 @@ program main_peps
                        CGA_level, CGPSA_level, omega_sor
     namelist /coefficients/ alpha_x, alpha_y, alpha_z
     namelist /gpu/ smoother, precision, coarse_switch, inner_cycles, max_refine, verbose, rhs_mode, seed, output_file, &
-                   write_solution, inner_nu, inner_nu_post, inner_omega
+                   write_solution, inner_nu, inner_nu_post, inner_omega, rhs_file
 
     smoother = 0; precision = 0; coarse_switch = 4096; inner_cycles = 2; max_refine = 30; verbose = 0
     inner_nu = 2; inner_nu_post = 0; inner_omega = 1.35d0
-    rhs_mode = 0; seed = 20210501_8; output_file = ''; write_solution = .false.
+    rhs_mode = 0; seed = 20210501_8; output_file = ''; rhs_file = ''; write_solution = .false.
     alpha_x = 1.0d0; alpha_y = 1.0d0; alpha_z = 1.0d0
 
     open(unit=1, file="PARA_INPUT.inp", status='old')
@@ -111,6 +114,11 @@ program main_peps
                           c_loc(s_domain%xg(0)), c_loc(s_domain%yg(0)), c_loc(s_domain%zg(0))), 'fill_rhs')
     endif
     print '(a)', '[Poisson] Geometry and rhs constructed.'
+    if (len_trim(rhs_file) > 0) then
+        open(newunit=u, file=trim(rhs_file), access='stream', form='unformatted', status='replace')
+        write(u) s_domain%b
+        close(u)
+    endif
 
     call matrix_heptadiagonal_create(a_poisson, s_domain)
     print '(a)', '[Poisson] Poisson matrix constructed.'
@@ -164,11 +172,30 @@ program main_peps
 
 contains
 
-    !> The reference's three solution files for one process (the gathers of
-    !> src/main_peps.f90:204-446 reduce to the subdomain's interior).
+    !> The reference's seven solution files for one process
+    !> (src/main_peps.f90:204-446).  Its point-to-point, gatherv and
+    !> gatherv-with-halo collections all reduce to the subdomain's interior on
+    !> one process, so each formatted file holds the same lines and each
+    !> unformatted file the same record (the reference's
+    !> solution_all_gatherv_halo.bin also writes the gatherv array, :397).
     subroutine file_write
+        integer(kind=4) :: uf
+        call write_formatted("solution_all.dat")              ! :304-313
+        call write_record("solution_all.bin")                 ! :315-317
+        call write_formatted("solution_all_gatherv.dat")      ! :346-355
+        call write_record("solution_all_gatherv.bin")         ! :357-359
+        call write_formatted("solution_all_gatherv_halo.dat") ! :385-394
+        call write_record("solution_all_gatherv_halo.bin")    ! :396-398
+        open(newunit=uf, form="unformatted", access='stream', file="solution_all_MPI_IO.bin", status='replace')
+        write(uf) s_domain%x(1:s_domain%nx, 1:s_domain%ny, 1:s_domain%nz)   ! :413-419
+        close(uf)
+    end subroutine file_write
+
+    !> One line (x, y, z, u) per interior cell, x fastest, format (3D20.10,1E30.20).
+    subroutine write_formatted(fname)
+        character(*), intent(in) :: fname
         integer(kind=4) :: i, j, k, uf
-        open(newunit=uf, form="formatted", file="solution_all.dat", status='replace')
+        open(newunit=uf, form="formatted", file=fname, status='replace')
         do k = 1, g_domain%nz
             do j = 1, g_domain%ny
                 do i = 1, g_domain%nx
@@ -177,13 +204,16 @@ contains
             enddo
         enddo
         close(uf)
-        open(newunit=uf, form="unformatted", file="solution_all.bin", status='replace')
+    end subroutine write_formatted
+
+    !> One unformatted sequential record of u(1:nx,1:ny,1:nz).
+    subroutine write_record(fname)
+        character(*), intent(in) :: fname
+        integer(kind=4) :: uf
+        open(newunit=uf, form="unformatted", file=fname, status='replace')
         write(uf) s_domain%x(1:s_domain%nx, 1:s_domain%ny, 1:s_domain%nz)
         close(uf)
-        open(newunit=uf, form="unformatted", access='stream', file="solution_all_MPI_IO.bin", status='replace')
-        write(uf) s_domain%x(1:s_domain%nx, 1:s_domain%ny, 1:s_domain%nz)
-        close(uf)
-    end subroutine file_write
+    end subroutine write_record
 
     !> Reads both binary files back and prints every cell where they differ
     !> (src/main_peps.f90:448-484).

@@ -84,6 +84,10 @@ contains
             print '(a)', '[Error] multigrid_create has not been called'
             stop 301
         endif
+        call pepsmg_check_padded(sol, sdm%nx, sdm%ny, sdm%nz, 'sol', 'multigrid_solve_vcycle')
+        call pepsmg_check_padded(rsd, sdm%nx, sdm%ny, sdm%nz, 'rsd', 'multigrid_solve_vcycle')
+        call pepsmg_check_padded(rhs, sdm%nx, sdm%ny, sdm%nz, 'rhs', 'multigrid_solve_vcycle')
+        call pepsmg_check_coeff(a_poisson, sdm%nx, sdm%ny, sdm%nz, 'multigrid_solve_vcycle')
         cap = max(n_vcycles, 1) + 64
         allocate(hist(cap), cit(cap))
         mg_last_stats%history_cap = cap
@@ -106,6 +110,12 @@ contains
                         hist(c) * sqrt(mg_last_stats%res0), sqrt(mg_last_stats%res0), hist(c)
                 enddo
             endif
+            ! The reference stops at number_of_vcycles without a word
+            ! (src/multigrid.f90:1928-1974); its own sample input caps the
+            ! solve at 3 cycles on purpose, so this is reported, not fatal.
+            if (mg_last_stats%converged == 0) print '(a,i5,a,e15.7,a,e15.7)', &
+                '[MG] Not converged: stopped at the cycle cap after ', mg_last_stats%cycles, &
+                ' V-cycles, rho = ', mg_last_stats%rho, ' >= tolerance = ', tolerance
         endif
         mg_last_stats%history = c_null_ptr
         mg_last_stats%coarse_iters = c_null_ptr

@@ -31,6 +31,35 @@ contains
         p = c_loc(a(0,0,0))
     end function pepsmg_ptr3
 
+    !> Stops with an [Error] line unless a has the padded extents
+    !> (0:nx+1,0:ny+1,0:nz+1) of the grid: the library copies that many
+    !> values in and out through the array's address.
+    subroutine pepsmg_check_padded(a, nx, ny, nz, name, where)
+        real(kind=8), intent(in), contiguous :: a(0:,0:,0:)
+        integer(kind=4), intent(in) :: nx, ny, nz
+        character(*), intent(in)    :: name, where
+        if (size(a, 1) /= nx + 2 .or. size(a, 2) /= ny + 2 .or. size(a, 3) /= nz + 2) then
+            print '(a,a,a,a,a,3i7,a,3i7)', '[Error] ', where, ': ', name, ' has extents', size(a, 1), size(a, 2), &
+                size(a, 3), ', expected', nx + 2, ny + 2, nz + 2
+            stop 301
+        endif
+    end subroutine pepsmg_check_padded
+
+    !> Stops with an [Error] line unless an allocated coefficient array is
+    !> coeff(0:6,nx,ny,nz).
+    subroutine pepsmg_check_coeff(a_poisson, nx, ny, nz, where)
+        type(matrix_heptadiagonal), intent(in) :: a_poisson
+        integer(kind=4), intent(in) :: nx, ny, nz
+        character(*), intent(in)    :: where
+        if (.not. allocated(a_poisson%coeff)) return
+        if (size(a_poisson%coeff, 1) /= 7 .or. size(a_poisson%coeff, 2) /= nx .or. &
+            size(a_poisson%coeff, 3) /= ny .or. size(a_poisson%coeff, 4) /= nz) then
+            print '(a,a,a,4i7)', '[Error] ', where, ': coeff has extents', size(a_poisson%coeff, 1), &
+                size(a_poisson%coeff, 2), size(a_poisson%coeff, 3), size(a_poisson%coeff, 4)
+            stop 301
+        endif
+    end subroutine pepsmg_check_coeff
+
     !> Address of a matrix's coefficient array (NULL when unallocated).
     function pepsmg_coeff_ptr(a_poisson) result(p)
         type(matrix_heptadiagonal), intent(in), target :: a_poisson

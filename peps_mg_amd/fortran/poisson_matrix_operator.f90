@@ -20,6 +20,8 @@ contains
         integer(kind=4), intent(in)           :: nx, ny, nz
         logical, intent(in)                   :: is_serial(0:2)
         real(c_double) :: r
+        call pepsmg_check_padded(x, nx, ny, nz, 'x', 'vv_dot_3d_matrix')
+        call pepsmg_check_padded(y, nx, ny, nz, 'y', 'vv_dot_3d_matrix')
         call pepsmg_check(pepsmg_vv_dot(nx, ny, nz, pepsmg_ptr3(x), pepsmg_ptr3(y), r), 'vv_dot_3d_matrix')
         result = r
     end subroutine vv_dot_3d_matrix
@@ -35,6 +37,9 @@ contains
         logical, intent(in)                               :: is_serial(0:2)
         type(pepsmg_grid_t) :: g
         g = pepsmg_grid_of(dm)
+        call pepsmg_check_padded(y, dm%nx, dm%ny, dm%nz, 'y', 'mv_mul_poisson_matrix')
+        call pepsmg_check_padded(x, dm%nx, dm%ny, dm%nz, 'x', 'mv_mul_poisson_matrix')
+        call pepsmg_check_coeff(a_poisson, dm%nx, dm%ny, dm%nz, 'mv_mul_poisson_matrix')
         call pepsmg_check(pepsmg_mv_mul(g, pepsmg_ptr3(y), pepsmg_coeff_ptr(a_poisson), pepsmg_ptr3(x)), &
                           'mv_mul_poisson_matrix')
     end subroutine mv_mul_poisson_matrix

@@ -72,10 +72,11 @@ class Multigrid:
     def solve_vcycle(self, sol, rsd, a_poisson, rhs, maxiteration, tolerance, omega_sor):
         """multigrid_solve_vcycle on host arrays; sol and rsd updated in place."""
         st, hist, cit = self._stats()
-        coeff = None if a_poisson is None else np.ascontiguousarray(a_poisson.coeff)
+        n = (self.sdm.nx, self.sdm.ny, self.sdm.nz)
         _lib.check(_lib.lib().pepsmg_multigrid_solve_vcycle(
-            self.h, _lib.dptr(sol), _lib.dptr(rsd), _lib.dptr(coeff), _lib.dptr(rhs), maxiteration,
-            tolerance, omega_sor, C.byref(st)))
+            self.h, _lib.padded_ptr(sol, *n, "sol"), None if rsd is None else _lib.padded_ptr(rsd, *n, "rsd"),
+            _lib.coeff_ptr(a_poisson, *n), _lib.padded_ptr(rhs, *n, "rhs"), maxiteration, tolerance, omega_sor,
+            C.byref(st)))
         return SolveResult(st, hist, cit)
 
     def solve_vcycle_device(self, d_sol, d_rsd, d_rhs, maxiteration, tolerance, omega_sor, stream=None):

@@ -147,29 +147,47 @@ def test_vcycle_cg_smoother():
     assert np.max(np.abs(xg - xo)) <= 1e-10 * np.abs(xo).max()
 
 
+def workload_rhs(g, seed):
+    """b = -(U[-1,1] + 3 pi^2 sin(pi x) sin(pi y) sin(pi z)), the workload's
+    right-hand side (U: the SplitMix64 field of the seed)."""
+    b = O.uniform_field(g.nx, g.ny, g.nz, seed)
+    s = (np.sin(np.pi * g.xg)[None, None, 1:-1] * np.sin(np.pi * g.yg)[None, 1:-1, None] *
+         np.sin(np.pi * g.zg)[1:-1, None, None])
+    b[1:-1, 1:-1, 1:-1] += 3 * np.pi ** 2 * s
+    return -b
+
+
 @pytest.mark.parametrize("inner", [dict(), dict(inner_nu=1, inner_omega=0.0, inner_cycles=4),
                                    dict(inner_nu=2, inner_nu_post=1), dict(inner_nu=1, inner_nu_post=3, inner_cycles=3)],
                          ids=["default", "nu1-omega1", "v21", "v13"])
 @pytest.mark.parametrize("tx,ty", [(0, 0), (64, 0), (128, 32), (64, 32)])
 @pytest.mark.parametrize("dims,lens", [((64, 64, 64), (1, 1, 1)), ((32, 32, 32), (1, 1, 2))])
-def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty, inner):
+@pytest.mark.parametrize("rhs,tol", [("workload", 1e-10), ("random", 1e-12)])
+def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty, inner, rhs, tol):
+    """Mixed precision within 1e-10 (max, relative) of the oracle's FP64
+    solution at the same tolerance.  A solution with relative residual below
+    tol differs from the exact one by at most 4e-11 (relative) on these grids
+    for the workload's right-hand side at tol 1e-10, and for a purely random
+    one at tol 1e-12 (at 1e-10 a random right-hand side's solution is only
+    ~1e-9 accurate, so that pair is tested at 1e-12); both figures measured
+    with the oracle against 1e-14 solves."""
     g = O.Grid(*dims, l=lens)
     A = g.matrix()
-    b = -O.uniform_field(*dims, 7)
+    b = workload_rhs(g, 7) if rhs == "workload" else -O.uniform_field(*dims, 7)
     nl = int(np.log2(dims[0] // 4))
     mo = O.Multigrid(g, nl, 80)
     xo, ro = O.padded(*dims), O.padded(*dims)
-    mo.solve_vcycle(xo, ro, A, b, 2, 1e-10, 1.0)
+    mo.solve_vcycle(xo, ro, A, b, 2, tol, 1.0)
     mg = Multigrid(sdm_from(g), nl, 80, precision=1, tile_x=tx, tile_y=ty, **inner)
     xg, rg = O.padded(*dims), O.padded(*dims)
-    res = mg.solve_vcycle(xg, rg, None, b, 2, 1e-10, 1.0)
-    assert res.converged and res.rho < 1e-10
+    res = mg.solve_vcycle(xg, rg, None, b, 2, tol, 1.0)
+    assert res.converged and res.rho < tol
     # the true FP64 residual of the returned solution
     K = RM.laplacian(g.dxm[1:-1], g.dym[1:-1], g.dzm[1:-1])
     r = RM.flat(b) - K @ RM.flat(xg)
-    assert np.linalg.norm(r) / np.sqrt(res.res0) < 1.01e-10
+    assert np.linalg.norm(r) / np.sqrt(res.res0) < 1.01 * tol
     err = np.abs(xg - xo).max() / np.abs(xo).max()
-    assert err < 1e-8, err
+    assert err <= 1e-10, err
 
 
 def _refinement_sweep(dims, lens, nlevel, tols):
@@ -366,3 +384,181 @@ def test_fp64_decompositions_agree_large(dims, nlevel):
         assert r.cycles == out[0][1].cycles
         np.testing.assert_allclose(r.history, out[0][1].history, rtol=1e-12, atol=0)
         assert torch.equal(x, out[0][0])
+
+
+# --- behaviours of the reference's cycle (src/multigrid.f90:1907-1976) -----
+
+def rand_guess(dims, seed):
+    """A non-zero initial guess with zero ghosts (homogeneous Dirichlet)."""
+    rng = np.random.default_rng(seed)
+    x = O.padded(*dims)
+    x[1:-1, 1:-1, 1:-1] = rng.uniform(-0.05, 0.05, dims[::-1])
+    return x
+
+
+def solve_pair(dims, lens, nlevel, ncycle, nu, omega, x0, b, tol=1e-10, **opts):
+    """The same solve on the oracle and the GPU (host entry point) from x0."""
+    g = O.Grid(*dims, l=lens)
+    A = g.matrix()
+    mo = O.Multigrid(g, nlevel, ncycle)
+    xo, ro = x0.copy(), O.padded(*dims)
+    nc, hist, cit, res0 = mo.solve_vcycle(xo, ro, A, b, nu, tol, omega)
+    mg = Multigrid(sdm_from(g), nlevel, ncycle, **opts)
+    xg, rg = x0.copy(), O.padded(*dims)
+    res = mg.solve_vcycle(xg, rg, matrix.MatrixHeptadiagonal(0, A), b, nu, tol, omega)
+    mg.destroy()
+    return (xo, ro, nc, hist, cit, res0), (xg, rg, res)
+
+
+FP64_PATHS = [dict(), dict(fused_smoother=0), dict(coarse_switch=0)]
+FP64_IDS = ["fused", "colour", "per-level-coarse"]
+
+
+@pytest.mark.parametrize("opts", FP64_PATHS, ids=FP64_IDS)
+@pytest.mark.parametrize("dims,lens,nlevel", [((64, 64, 64), (1, 1, 1), 4), ((32, 32, 32), (1, 1, 2), 3)])
+def test_nonzero_initial_guess_fp64_bitwise(dims, lens, nlevel, opts):
+    """rho is relative to |b - A x0| of the incoming sol (:1924-1925); with
+    the fused smoother that norm comes out of the first smoothing pass."""
+    g = O.Grid(*dims, l=lens)
+    x0 = rand_guess(dims, 4)
+    b = workload_rhs(g, 3)
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = solve_pair(dims, lens, nlevel, 80, 2, 1.0, x0, b, **opts)
+    assert res.converged and res.cycles == nc
+    assert abs(res.res0 - res0) <= 1e-13 * res0
+    assert np.max(np.abs(res.history - hist)) <= 1e-12
+    assert np.array_equal(res.coarse_iters, cit)
+    assert np.array_equal(xg, xo) and np.array_equal(rg, ro)
+
+
+@pytest.mark.parametrize("dims,lens,nlevel", [((64, 64, 64), (1, 1, 1), 4), ((32, 32, 32), (1, 1, 2), 3)])
+def test_nonzero_initial_guess_mixed(dims, lens, nlevel):
+    """The first refinement step reads the non-zero sol from the caller's
+    array (k_refine_start); the result is within 1e-10 of the oracle's FP64
+    solve from the same x0, and res0 is the same norm."""
+    g = O.Grid(*dims, l=lens)
+    x0 = rand_guess(dims, 5)
+    b = workload_rhs(g, 3)
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = solve_pair(dims, lens, nlevel, 80, 2, 1.0, x0, b, precision=1)
+    assert res.converged and res.rho < 1e-10
+    assert abs(res.res0 - res0) <= 1e-12 * res0
+    assert np.abs(xg - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("opts", FP64_PATHS, ids=FP64_IDS)
+@pytest.mark.parametrize("ncycle", [1, 3])
+def test_cycle_cap_fp64_bitwise(ncycle, opts):
+    """number_of_vcycles below what the tolerance needs: the loop ends after
+    ncycle cycles (:1928) without meeting rho < tol; converged = 0 and the
+    state equals the oracle's after the same cycles."""
+    dims = (64, 64, 64)
+    g = O.Grid(*dims)
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = solve_pair(dims, (1, 1, 1), 4, ncycle, 2, 1.0, O.padded(*dims),
+                                                              workload_rhs(g, 8), **opts)
+    assert nc == ncycle and res.cycles == ncycle and not res.converged and res.rho > 1e-10
+    assert np.max(np.abs(res.history - hist)) <= 1e-12
+    assert np.array_equal(res.coarse_iters, cit)
+    assert np.array_equal(xg, xo) and np.array_equal(rg, ro)
+
+
+@pytest.mark.parametrize("ncycle,inner", [(3, 2), (5, 2), (4, 3)])
+def test_cycle_cap_mixed(ncycle, inner):
+    """In mixed precision number_of_vcycles caps the FP32 V-cycles of the
+    whole solve: the last refinement step runs only the cycles that are
+    left, so exactly ncycle run; the reported rho is the true FP64 relative
+    residual of the returned sol."""
+    dims = (48, 48, 48)
+    g = O.Grid(*dims)
+    b = workload_rhs(g, 9)
+    mg = Multigrid(sdm_from(g), 4, ncycle, precision=1, inner_cycles=inner)
+    xg, rg = O.padded(*dims), O.padded(*dims)
+    res = mg.solve_vcycle(xg, rg, None, b, 2, 1e-10, 1.0)
+    assert res.cycles == ncycle and not res.converged
+    assert res.refine_steps == -(-ncycle // inner)
+    K = RM.laplacian(g.dxm[1:-1], g.dym[1:-1], g.dzm[1:-1])
+    r = RM.flat(b) - K @ RM.flat(xg)
+    assert abs(np.linalg.norm(r) / np.sqrt(res.res0) - res.rho) <= 1e-6 * res.rho
+    assert np.abs(RM.flat(rg) - r).max() <= 1e-11 * np.abs(b).max()
+    mg.destroy()
+
+
+@pytest.mark.parametrize("dims,nlevel", [((32, 32, 32), 3), ((16, 16, 16), 4)])
+def test_maxiteration_zero_bitwise(dims, nlevel):
+    """maxiteration = 0: no smoothing, the cycle is the coarse correction
+    alone (the fused smoother's norm fold needs nu > 0, so the norms come
+    from separate residual passes)."""
+    g = O.Grid(*dims)
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = solve_pair(dims, (1, 1, 1), nlevel, 6, 0, 1.0, O.padded(*dims),
+                                                              workload_rhs(g, 2))
+    assert res.cycles == nc
+    assert abs(res.res0 - res0) <= 1e-13 * res0
+    assert np.max(np.abs(res.history - hist)) <= 1e-12
+    assert np.array_equal(res.coarse_iters, cit)
+    assert np.array_equal(xg, xo) and np.array_equal(rg, ro)
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(fused_smoother=0), dict(precision=1)], ids=["fused", "colour", "mixed"])
+def test_zero_rhs_returns_zero(opts):
+    """b = 0 from x0 = 0: x0 is exact.  The reference would divide 0 by 0
+    and run every cycle; here the solve reports converged with rho = 0 and
+    returns sol = rsd = 0."""
+    import torch
+    dims = (32, 32, 32)
+    g = O.Grid(*dims)
+    mg = Multigrid(sdm_from(g), 3, 50, **opts)
+    x, r = O.padded(*dims), O.padded(*dims)
+    res = mg.solve_vcycle(x, r, None, O.padded(*dims), 2, 1e-10, 1.0)
+    assert res.converged and res.rho == 0.0 and res.res0 == 0.0
+    assert not x.any() and not r.any()
+    d_b = torch.zeros((34, 34, 34), dtype=torch.float64, device="cuda")
+    d_x = torch.zeros_like(d_b)
+    res = mg.solve_vcycle_device(d_x.data_ptr(), 0, d_b.data_ptr(), 2, 1e-10, 1.0)
+    assert res.converged and res.rho == 0.0 and not d_x.any().item()
+    mg.destroy()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("where,value", [("rhs", np.nan), ("rhs", np.inf), ("sol", np.nan)])
+def test_nonfinite_input_rejected(precision, where, value):
+    """A NaN or infinity in rhs or in the interior of sol makes |b - A x0|
+    non-finite: PEPSMG_ERR_ARG, and (device entry point) sol is left as it
+    was."""
+    import torch
+    dims = (32, 32, 32)
+    g = O.Grid(*dims)
+    mg = Multigrid(sdm_from(g), 3, 50, precision=precision)
+    b = workload_rhs(g, 1)
+    x = O.padded(*dims)
+    (b if where == "rhs" else x)[9, 10, 11] = value
+    with pytest.raises(_lib.PepsmgError) as e:
+        mg.solve_vcycle(x.copy(), O.padded(*dims), None, b, 2, 1e-10, 1.0)
+    assert e.value.code == _lib.ERR_ARG
+    d_b = torch.from_numpy(b).cuda()
+    d_x = torch.from_numpy(x).cuda()
+    with pytest.raises(_lib.PepsmgError) as e:
+        mg.solve_vcycle_device(d_x.data_ptr(), 0, d_b.data_ptr(), 2, 1e-10, 1.0)
+    assert e.value.code == _lib.ERR_ARG
+    got = d_x.cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(x)) and np.array_equal(got[~np.isnan(x)], x[~np.isnan(x)])
+    # the context is usable afterwards
+    x = O.padded(*dims)
+    res = mg.solve_vcycle(x, O.padded(*dims), None, workload_rhs(g, 1), 2, 1e-10, 1.0)
+    assert res.converged
+    mg.destroy()
+
+
+def test_solve_rejects_wrong_array_shapes():
+    """The host entry point copies (nx+2)(ny+2)(nz+2) values through each
+    pointer: interior-only or transposed arrays are refused before any copy."""
+    dims = (16, 12, 8)
+    g = O.Grid(*dims)
+    mg = Multigrid(sdm_from(g), 2, 10)
+    good = O.padded(*dims)
+    for bad in (np.zeros((8, 12, 16)), np.zeros((18, 14, 10)), np.zeros((10, 14, 18), dtype=np.float32),
+                np.zeros((10, 14, 36))[:, :, ::2]):
+        with pytest.raises(_lib.PepsmgError) as e:
+            mg.solve_vcycle(bad, good.copy(), None, good, 2, 1e-10, 1.0)
+        assert e.value.code == _lib.ERR_ARG
+    with pytest.raises(_lib.PepsmgError):
+        mg.solve_vcycle(good.copy(), good.copy(), matrix.MatrixHeptadiagonal(0, np.zeros((8, 12, 16, 6))), good,
+                        2, 1e-10, 1.0)
+    mg.destroy()
