@@ -70,11 +70,30 @@ def oracle_solve(g, b, nlevel, ncycle, nu, omega, tol=1e-10, aggr=0, cga=0, smoo
     return x, r, nc, hist, cit, res0
 
 
-def assert_same_history(res, nc, hist, cit, res0):
+def sum_rounding(n):
+    """Relative rounding allowance for a sum of n squares evaluated in two
+    different orders (the oracle's sequential k, j, i loop, the GPU's tree):
+    each differs from the exact sum by about sqrt(n) eps in a random-walk
+    model; 4 sqrt(n) eps covers the difference of two such sums."""
+    return 4.0 * np.sqrt(n) * np.finfo(np.float64).eps / 2
+
+
+def rho_close(hist_gpu, hist_ref, n):
+    """|drho| <= 1e-12 per cycle, or the rounding of the two norm sums that
+    form rho = sqrt(|r|^2 / |r0|^2) where that is larger (rho above about
+    0.25 at 256^3: the CG(4) smoother's first cycle raises the residual to
+    rho = 1.8, whose 1e-12 absolute is 6e-13 relative, below the rounding of
+    a 1.7e7-term sum)."""
+    d = np.abs(np.asarray(hist_gpu) - np.asarray(hist_ref))
+    allow = np.maximum(1e-12, sum_rounding(n) * np.abs(hist_ref))
+    assert np.all(d <= allow), (d, allow)
+
+
+def assert_same_history(res, nc, hist, cit, res0, n):
     assert res.cycles == nc
-    assert abs(res.res0 - res0) <= 1e-13 * res0
-    assert np.max(np.abs(res.history - hist)) <= 1e-12, np.max(np.abs(res.history - hist))
+    rho_close(res.history, hist, n)
     assert np.array_equal(res.coarse_iters, cit)
+    assert abs(res.res0 - res0) <= sum_rounding(n) * res0, (res.res0, res0)
 
 
 def test_fp64_512_two_cycles_bitwise():
@@ -84,7 +103,7 @@ def test_fp64_512_two_cycles_bitwise():
     x, r, res = gpu_solve(g, b, 7, 2, 2, 1.0)
     xo, ro, nc, hist, cit, res0 = oracle_solve(g, b, 7, 2, 2, 1.0)
     assert nc == 2 and not res.converged
-    assert_same_history(res, nc, hist, cit, res0)
+    assert_same_history(res, nc, hist, cit, res0, n ** 3)
     assert np.array_equal(x, xo), np.abs(x - xo).max()
     assert np.array_equal(r, ro), np.abs(r - ro).max()
 
@@ -96,7 +115,7 @@ def test_fp64_256_rbgs_full_solve_bitwise():
     x, r, res = gpu_solve(g, b, 6, 200, 2, 1.0)
     xo, ro, nc, hist, cit, res0 = oracle_solve(g, b, 6, 200, 2, 1.0)
     assert res.converged and res.rho < 1e-10
-    assert_same_history(res, nc, hist, cit, res0)
+    assert_same_history(res, nc, hist, cit, res0, n ** 3)
     assert np.array_equal(x, xo), np.abs(x - xo).max()
     assert np.array_equal(r, ro)
 
@@ -105,14 +124,15 @@ def test_fp64_256_rbgs_full_solve_bitwise():
 def test_fp64_256_cg4_full_solve(persist):
     """CG(4) smoother (src/cg_poisson_matrix.f90:157-241): its dot products
     are summed in another order than the oracle's sequential loop, so the
-    iterates differ by rounding; cycle counts and rho agree, x to 1e-10."""
+    iterates differ by rounding; cycle counts agree, rho to rho_close's
+    bound, x to 1e-10."""
     n = 256
     g = O.Grid(n, n, n)
     b = workload_rhs(g)
     x, r, res = gpu_solve(g, b, 6, 200, 4, 1.0, smoother=1, coarse_switch=persist)
     xo, ro, nc, hist, cit, res0 = oracle_solve(g, b, 6, 200, 4, 1.0, smoother=1)
     assert res.converged and res.cycles == nc
-    assert np.max(np.abs(res.history - hist)) <= 1e-12
+    rho_close(res.history, hist, n ** 3)
     assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
 
 
@@ -123,7 +143,7 @@ def test_512_aniso_fp64_bitwise_and_mixed_within_1e10():
     xo, ro, nc, hist, cit, res0 = oracle_solve(g, b, 7, 200, 2, 1.0)
     x, r, res = gpu_solve(g, b, 7, 200, 2, 1.0)
     assert res.converged
-    assert_same_history(res, nc, hist, cit, res0)
+    assert_same_history(res, nc, hist, cit, res0, n ** 3)
     assert np.array_equal(x, xo)
     del x, r
     xm, rm, resm = gpu_solve(g, b, 7, 200, 2, 1.0, precision=1)
@@ -155,6 +175,6 @@ def test_reference_sample_input_shape():
     x, r, res = gpu_solve(g, b, 11, 3, 1, 1.6, aggr=(2, 6, (9, 6, 8)))
     xo, ro, nc, hist, cit, res0 = oracle_solve(g, b, 11, 3, 1, 1.6, aggr=2, cga=6)
     assert nc == 3 and res.cycles == 3 and not res.converged
-    assert_same_history(res, nc, hist, cit, res0)
+    assert_same_history(res, nc, hist, cit, res0, nx * ny * nz)
     assert np.array_equal(x, xo), np.abs(x - xo).max()
     assert np.array_equal(r, ro)
