@@ -163,6 +163,20 @@ int pepsmg_mv_mul(const pepsmg_grid *grid, double *y, const double *coeff, const
 int pepsmg_vv_dot(int32_t nx, int32_t ny, int32_t nz, const double *x, const double *y,
                   double *result);
 
+/* No counterpart in the reference: pepsmg_vv_dot for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  x and y are padded device arrays
+ * (0:nx+1,0:ny+1,0:nz+1); the interior dot product is written to the device
+ * double *d_result on `stream` (NULL: the default stream), without waiting
+ * for it.  The sum is bit-identical to pepsmg_vv_dot's for the same arrays.
+ * Allocates nothing: its partial sums live in a static 8 KiB device array
+ * per GPU (1024 doubles, enough for any grid of at most 2^31-1 rows of
+ * cells; larger grids are rejected with PEPSMG_ERR_ARG).  Every call on a
+ * GPU shares that array, so calls that overlap on one GPU (other streams,
+ * other host threads) race. */
+int pepsmg_vv_dot_device_noalloc(int32_t nx, int32_t ny, int32_t nz, const double *d_x,
+                                 const double *d_y, double *d_result, void *stream);
+
 /* Measurement hook (no counterpart in the reference): launches the
  * fine-level smoother kernel `reps` times on the context's level-0 arrays
  * (precision 0: FP64, 1: FP32) on the context's stream, timing them with

@@ -68,6 +68,8 @@ SIGNATURES = {
     "pepsmg_cg_solver": (C.c_int, [C.POINTER(Grid), _dp, _dp, _dp, C.c_int32, C.c_double, _ip, _dp]),
     "pepsmg_mv_mul": (C.c_int, [C.POINTER(Grid), _dp, _dp, _dp]),
     "pepsmg_vv_dot": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),
+    "pepsmg_vv_dot_device_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p]),
     "pepsmg_bench_smoother": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp, _dp,
                                         C.POINTER(C.c_char_p)]),
     "pepsmg_bench_stream": (C.c_int, [C.c_int64, C.c_int32, _dp, _dp]),

@@ -1048,6 +1048,76 @@ int pepsmg_mv_mul(const pepsmg_grid* grid, double* y, const double* coeff, const
   return o.download(y, o.d.r, 1);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scratch of pepsmg_vv_dot_device_noalloc: the per-block partial sums of one
+// dot product (RED_BLOCKS doubles, 8 KiB per GPU, held until the process
+// ends).  Every block writes its slot before the final reduction reads it,
+// so nothing needs clearing between calls; calls that overlap on one GPU
+// share it and race.  Host code never uses the symbol's name as an address:
+// noalloc_scratch() returns the device address hipGetSymbolAddress gives for
+// the current device.
+namespace pepsmg {
+
+constexpr int NOALLOC_SCRATCH_DOUBLES = RED_BLOCKS;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+// Device address of scratch on the current device, looked up once per device
+// (with that device current) and kept.
+static int noalloc_scratch(double** out) {
+  static std::mutex mu;
+  static double* addr[NOALLOC_MAX_DEVICES] = {};
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(PEPSMG_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    addr[dev] = (double*)p;
+  }
+  *out = addr[dev];
+  return PEPSMG_OK;
+}
+
+}  // namespace pepsmg
+
+extern "C" {
+
+int pepsmg_vv_dot_device_noalloc(int32_t nx, int32_t ny, int32_t nz, const double* d_x, const double* d_y,
+                                 double* d_result, void* stream) {
+  if (!d_x || !d_y || !d_result) return fail(PEPSMG_ERR_ARG, "NULL argument");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(PEPSMG_ERR_ARG, "grid %d x %d x %d is empty", nx, ny, nz);
+  // rows (j, k) are numbered in int by the reduction kernel
+  if ((long)ny * nz > 0x7fffffffL || nx > 0x7ffffffd)
+    return fail(PEPSMG_ERR_ARG, "grid %d x %d x %d exceeds the 2^31-1 rows of cells the reduction numbers", nx, ny, nz);
+  Lvl<double> L{};
+  L.nx = nx;
+  L.ny = ny;
+  L.nz = nz;
+  L.px = (long)nx + 2;
+  L.pxy = L.px * ((long)ny + 2);
+  const long o = 1 + L.px + L.pxy;   // cell (1,1,1) of the padded array
+  // the block count of pepsmg_vv_dot (Ops::red_blocks), so both sum in the same order
+  const long cells = (long)nx * ny * nz;
+  const int nb = (int)std::max(1L, std::min<long>((cells + RED_THREADS - 1) / RED_THREADS, RED_BLOCKS));
+  if (nb > NOALLOC_SCRATCH_DOUBLES) return fail(PEPSMG_ERR_ARG, "%d partial sums exceed the scratch", nb);
+  double* part = nullptr;
+  int rc = noalloc_scratch(&part);
+  if (rc) return rc;
+  (void)hipGetLastError();   // an error the caller's earlier HIP calls left is not this call's
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL((k_dot_partial<double, 0, false>), dim3(nb), dim3(RED_THREADS), 0, s, d_x + o, d_y + o, L, part,
+                     (double*)nullptr);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, s, part, nb, d_result);
+  HIP_TRY(hipGetLastError());   // a launch the runtime rejected
+  return PEPSMG_OK;
+}
+
 int pepsmg_vv_dot(int32_t nx, int32_t ny, int32_t nz, const double* x, const double* y, double* result) {
   if (nx < 1 || ny < 1 || nz < 1 || !x || !y || !result) return fail(PEPSMG_ERR_ARG, "invalid argument");
   std::vector<double> ones[3];
