@@ -13,9 +13,11 @@ the autotuner before the timed region); the FP64 solve is timed the same
 way and reported beside it, together with two runs that separate the gain of
 precision from that of the inner schedule (FP64 with the inner cycles' SOR
 factor; mixed with plain V(2,2) omega = 1 inner cycles).  The roofline
-object is for the fine-level smoother kernel, timed with HIP events on the
-stream it is launched on, next to a rocprofv3 kernel trace of the same
-launches (child process) and its HBM traffic from two counter passes.
+object is for the fine-level smoother kernel: every launch of it on the
+512^3 level inside the timed FP64 solves is bracketed by HIP events on the
+solve stream (the library's measurement hook); beside that, the same kernel
+launched back to back, a rocprofv3 kernel trace of those launches (child
+process) and its HBM traffic from two counter passes.
 
 The default run also carries the other workloads (each in one key):
 aniso_512 (the 512^3 grid with hx:hy:hz = 1:1:2, FP64 and mixed),
@@ -260,15 +262,34 @@ def workload_rhs_device(n, lz, sdm, torch):
     return d_b
 
 
-def timed_solves(mg, d_x, d_b, steps, warmup, label, torch, nu=2, omega=1.0):
+def kernel_timing(mg, enable):
+    """The library's measurement hook: HIP events on the solve stream around
+    every plain fused smoother launch on the finest level of the solves that
+    follow; returns (mean ms, launches, kernel) when switched off."""
+    from peps_mg_amd import _lib
+    L = _lib.lib()
+    if enable:
+        _lib.check(L.pepsmg_kernel_timing(mg.h, 1))
+        return None
+    _lib.check(L.pepsmg_kernel_timing(mg.h, 0))
+    tot, cnt, name = C.c_double(0), C.c_int32(0), C.c_char_p()
+    _lib.check(L.pepsmg_kernel_timing_read(mg.h, C.byref(tot), C.byref(cnt), C.byref(name)))
+    return (tot.value / cnt.value if cnt.value else None), cnt.value, name.value.decode()
+
+
+def timed_solves(mg, d_x, d_b, steps, warmup, label, torch, nu=2, omega=1.0, time_smoother=False):
     """Mean wall time of `steps` solves from x = 0 (after `warmup` untimed
-    ones), each bracketed by device synchronisation."""
+    ones), each bracketed by device synchronisation.  time_smoother: the
+    fine-level smoother launches of the timed solves are also timed with HIP
+    events (kernel_timing), read back after the last one."""
     stream = torch.cuda.current_stream().cuda_stream
     res = None
     for _ in range(warmup):
         d_x.zero_()
         res = mg.solve_vcycle_device(d_x.data_ptr(), 0, d_b.data_ptr(), nu, 1e-10, omega, stream)
     torch.cuda.synchronize()
+    if time_smoother:
+        kernel_timing(mg, True)
     t_all = time.perf_counter()
     for q in range(steps):
         t0 = time.perf_counter()
@@ -278,7 +299,11 @@ def timed_solves(mg, d_x, d_b, steps, warmup, label, torch, nu=2, omega=1.0):
         log(f"[bench] {label} step {q}: {(time.perf_counter() - t0) * 1e3:.2f} ms, cycles={res.cycles}, "
             f"rho={res.rho:.3e}")
     torch.cuda.synchronize()
-    return (time.perf_counter() - t_all) / steps, res
+    t = (time.perf_counter() - t_all) / steps
+    if time_smoother:
+        res.smoother = kernel_timing(mg, False)
+        log(f"[bench] {label}: fine-level smoother {res.smoother}")
+    return t, res
 
 
 def oracle_solve(n, lz, b, cycles=200):
@@ -674,13 +699,13 @@ def main():
 
     log(f"[bench] {n}^3 lz={lz} levels={levels}")
     mg64 = Multigrid(sdm, levels, 200)
-    t64, r64 = timed_solves(mg64, d_x, d_b, a.steps, a.warmup, "fp64", torch)
+    t64, r64 = timed_solves(mg64, d_x, d_b, a.steps, a.warmup, "fp64", torch, time_smoother=True)
     x64 = d_x.cpu().numpy()
     kw = dict(precision=1)
     if a.inner:
         kw["inner_cycles"] = a.inner
     mgmx = Multigrid(sdm, levels, 200, **kw)
-    tmx_def, rmx_def = timed_solves(mgmx, d_x, d_b, a.steps, a.warmup, "mixed", torch)
+    tmx_def, rmx_def = timed_solves(mgmx, d_x, d_b, a.steps, a.warmup, "mixed", torch, time_smoother=True)
     # precision alone: mixed with the FP64 solve's own V(2,2) omega = 1 cycles,
     # and FP64 with the inner cycles' SOR factor 1.35
     mgp = Multigrid(sdm, levels, 200, precision=1, inner_omega=0.0, inner_nu=2, inner_nu_post=2)
@@ -711,8 +736,12 @@ def main():
                                                     C.byref(kn)))
         return avg.value, nbytes.value, kn.value.decode()   # (the library reuses its name buffer)
 
-    avg64, nbytes64, kname64 = smoother(mg64, 0)
-    avg32, nbytes32, kname32 = smoother(mgmx, 1)
+    bb64, nbytes64, kname64 = smoother(mg64, 0)
+    bb32, nbytes32, kname32 = smoother(mgmx, 1)
+    # the fine-level smoother inside the timed solves (the library's event hook)
+    avg64, n64, kin64 = r64.smoother
+    avg32, n32, kin32 = rmx_def.smoother
+    assert kin64 == kname64 and kin32 == kname32, (kin64, kname64, kin32, kname32)
     achieved = nbytes64 / (avg64 * 1e-3) / 1e9
     achieved32 = nbytes32 / (avg32 * 1e-3) / 1e9
     # the bandwidth a kernel with the smoother's streams (x, b in; x out) and no
@@ -778,25 +807,38 @@ def main():
              f"inner cycles, SOR {tuned.get('inner_omega', 1.35):g}, {tuned.get('inner_cycles', 2)} per FP64 "
              f"refinement step (autotuned)" if tuned else
              "mixed: FP32 V(2,2) inner cycles, SOR 1.35, 2 per FP64 refinement step (defaults)")
+    def frac(nbytes, ms):
+        return round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": round(traffic) if traffic else None, "traffic_note": traffic_note,
             "traffic_read": round(tr["FETCH_SIZE"]) if tr else None,
             "traffic_write": round(tr["WRITE_SIZE"]) if tr else None,
-            "kernel": kname64, "avg_ms": avg64, "bytes_per_launch": nbytes64,
-            "stream_ceiling_gbs": round(stream_gbs, 1), "frac_of_stream_ceiling": round(achieved / stream_gbs, 4)}
+            "kernel": kname64, "avg_ms": avg64, "launches": n64, "bytes_per_launch": nbytes64,
+            "timing": "HIP events on the solve stream around every launch of the kernel on the 512^3 level "
+                      "inside the timed FP64 solves",
+            "stream_ceiling_gbs": round(stream_gbs, 1), "frac_of_stream_ceiling": round(achieved / stream_gbs, 4),
+            "back_to_back": {"avg_ms": bb64, "frac": frac(nbytes64, bb64),
+                             "note": f"{a.smoother_reps} launches in a row after {a.smoother_reps} untimed ones "
+                                     "(pepsmg_bench_smoother), events around the whole run"}}
     roof32 = {"achieved": round(achieved32, 1), "frac": round(achieved32 / HBM_PEAK_GBS, 4), "kernel": kname32,
-              "avg_ms": avg32, "bytes_per_launch": nbytes32}
+              "avg_ms": avg32, "launches": n32, "bytes_per_launch": nbytes32,
+              "timing": "HIP events around every launch on the 512^3 level inside the timed mixed solves "
+                        "(default options)",
+              "back_to_back": {"avg_ms": bb32, "frac": frac(nbytes32, bb32)}}
     if trace:
         t64r, t32r = trace[kname64], trace[kname32]
-        roof.update({"trace_avg_ms": t64r["trace_avg_ms"], "trace_event_avg_ms": t64r["event_avg_ms_same_run"],
-                     "trace_span_per_launch_ms": t64r["trace_span_per_launch_ms"],
-                     "trace_avg_all_dispatches_ms": t64r["trace_avg_all_ms"],
-                     "frac_trace": round(nbytes64 / (t64r["trace_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "trace_note": f"rocprofv3 --kernel-trace of the {a.smoother_reps} event-timed launches in a "
-                                   "child run of --kernels-only (trace_event_avg_ms: the HIP events of that run)"})
-        roof32.update({"trace_avg_ms": t32r["trace_avg_ms"], "trace_event_avg_ms": t32r["event_avg_ms_same_run"],
-                       "frac_trace": round(nbytes32 / (t32r["trace_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+        roof["back_to_back"].update({
+            "trace_avg_ms": t64r["trace_avg_ms"], "trace_event_avg_ms": t64r["event_avg_ms_same_run"],
+            "trace_span_per_launch_ms": t64r["trace_span_per_launch_ms"],
+            "trace_avg_all_dispatches_ms": t64r["trace_avg_all_ms"],
+            "frac_trace": frac(nbytes64, t64r["trace_avg_ms"]),
+            "trace_note": f"rocprofv3 --kernel-trace of the {a.smoother_reps} event-timed launches in a child run of "
+                          "--kernels-only (trace_event_avg_ms: the HIP events of that run)"})
+        roof32["back_to_back"].update({"trace_avg_ms": t32r["trace_avg_ms"],
+                                       "trace_event_avg_ms": t32r["event_avg_ms_same_run"],
+                                       "frac_trace": frac(nbytes32, t32r["trace_avg_ms"])})
     elif not a.no_traffic:
         roof["trace_note"] = trace_note
     out = {

@@ -187,6 +187,19 @@ int pepsmg_vv_dot_device_noalloc(int32_t nx, int32_t ny, int32_t nz, const doubl
 int pepsmg_bench_smoother(pepsmg_ctx *ctx, int32_t precision, int32_t reps, double omega,
                           double *avg_ms, double *bytes, const char **kernel_name);
 
+/* Measurement hook (no counterpart in the reference): enable = 1 makes every
+ * following solve on ctx bracket the plain red/black smoother launches on its
+ * finest level (FP64; in precision 1 the FP32 smoother) with HIP events on
+ * the solve stream, and clears the totals; 0 stops.  Up to 256 launches per
+ * solve are timed. */
+int pepsmg_kernel_timing(pepsmg_ctx *ctx, int32_t enable);
+
+/* Totals since the last pepsmg_kernel_timing(ctx, 1) (kept when timing is
+ * switched off): summed duration of the timed launches (ms), their number,
+ * and the kernel's symbol (valid until timing is next switched on). */
+int pepsmg_kernel_timing_read(const pepsmg_ctx *ctx, double *total_ms, int32_t *launches,
+                              const char **kernel_name);
+
 /* Measurement hook (no counterpart in the reference): the bandwidth a
  * kernel with the smoother's access pattern but no stencil reaches, i.e.
  * y = x/2 + b over n doubles (two streams read, one written, 16 bytes per

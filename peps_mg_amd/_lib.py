@@ -73,6 +73,8 @@ SIGNATURES = {
     "pepsmg_bench_smoother": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, _dp, _dp,
                                         C.POINTER(C.c_char_p)]),
     "pepsmg_bench_stream": (C.c_int, [C.c_int64, C.c_int32, _dp, _dp]),
+    "pepsmg_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pepsmg_kernel_timing_read": (C.c_int, [C.c_void_p, _dp, _ip, C.POINTER(C.c_char_p)]),
     "pepsmg_fill_rhs_device": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_uint64, C.c_double,
                                          C.c_double, _dp, _dp, _dp, C.c_void_p]),
     "pepsmg_fill_rhs_host": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _dp, C.c_uint64, C.c_double, C.c_double,

@@ -99,7 +99,8 @@ kb_rb_fused(const FusedItem<T>* __restrict__ items, int n, const int* __restrict
   const int q = batch_find(items, n, blockIdx.x);
   const FusedItem<T>& it = items[q];
   if (!active[it.item]) return;
-  rb_fused_block<T, TX, TY, MODE, PF>(blockIdx.x - it.block0, it.xo, it.xi, it.b, it.L, omega, it.G, nullptr);
+  rb_fused_block<T, TX, TY, MODE, PF>(blockIdx.x - it.block0, it.xo, it.xi, it.b, it.L, omega, it.G, nullptr, it.L,
+                                      (T*)nullptr);
 }
 
 template <typename T>

@@ -35,6 +35,15 @@
 // starts from): the residual of plane m is evaluated in the red phase of
 // step m, when x_in planes m-1 .. m+1 are in LDS, with mv_mul's expression
 // (src/poisson_matrix_operator.f90:150-156).
+// FP64 only, the ends of a solve on the caller's padded (0:n+1)^3 arrays
+// (layout U: pitches nx+2 and (nx+2)(ny+2), base at cell (1,1,1); their
+// pairs (i odd, i+1) are only 8-byte aligned, accessed as unaligned 16-byte
+// vectors):
+// MODE 3: MODE 2 with x_in and b read from the caller's arrays, and each
+//   tile's b stored into the solver's layout (ext), so the solve needs no
+//   copy-in pass.
+// MODE 4: MODE 0 that also stores x_out into the caller's array (ext), for
+//   the iteration expected to end the solve (no copy-out pass).
 #pragma once
 #include <type_traits>
 
@@ -64,6 +73,26 @@ __device__ __forceinline__ typename Vec2<T>::type ld_stream_v2(const T* p) {
   o.x = r.x;
   o.y = r.y;
   return o;
+}
+
+// Two adjacent elements at an address aligned to one element only (the
+// pairs (i odd, i+1) of the caller's padded rows): one 16-byte (FP64) access,
+// which gfx950 performs unaligned, instead of two element accesses that each
+// cover every other element of a cache line.
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type ld_pair_unal(const T* p) {
+  typedef T v2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+  const v2 r = *reinterpret_cast<const v2*>(p);
+  typename Vec2<T>::type o;
+  o.x = r.x;
+  o.y = r.y;
+  return o;
+}
+template <typename T>
+__device__ __forceinline__ void st_pair_unal(T* p, T a, T b) {
+  typedef T v2 __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+  v2 v = {a, b};
+  *reinterpret_cast<v2*>(p) = v;
 }
 
 struct FusedGrid {
@@ -105,7 +134,11 @@ struct FusedShape {
 template <typename T, int TX, int TY, int MODE, int PF>
 __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __restrict__ xi,
                                                 const T* __restrict__ b, const Lvl<T>& L, T omega, int i0, int j0,
-                                                int k0, int k1, T* __restrict__ sx, T* __restrict__ sr) {
+                                                int k0, int k1, T* __restrict__ sx, T* __restrict__ sr,
+                                                const Lvl<T>& U, T* __restrict__ ext) {
+  constexpr bool NORM = MODE == 2 || MODE == 3;   // partial sums of (b - A x_in)^2
+  constexpr bool CALLER_IN = MODE == 3;           // x_in and b in the caller's layout U
+  constexpr bool CALLER_OUT = MODE == 4;          // x_out also into the caller's layout U
   using S = FusedShape<T, TX, TY>;
   constexpr int NT = S::NT, RP = S::RP, QY = S::QY, TP = S::TP;
   constexpr int NXL = S::NXL, NRED = S::NRED, NBLK = S::NBLK, PLANE = S::PLANE;
@@ -114,6 +147,9 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   using V2 = typename Vec2<T>::type;
   const int t = threadIdx.x;
   const long pxy = L.pxy;
+  // pitches of the inputs x_in and b
+  const int ipx = CALLER_IN ? (int)U.px : (int)L.px;
+  const long ipxy = CALLER_IN ? U.pxy : L.pxy;
 
   // ---- red pairs: ring row qy (y = j0-1+qy), pair p (x = i0-2+2p, +1) ----
   bool r_in[NRED][2];
@@ -145,14 +181,17 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
       r_s4[q][z] = c1 + c2 + c3 + c4;
       r_ss[q][z] = (first ? 0 : SB) + rrow;
       r_oo[q][z] = (first ? SB : 0) + rrow - (first ? 1 : 0);   // west neighbour; east is +1
-      r_bo[q][z] = in ? (gj - 1) * (int)L.px + (ic - 1) : 0;
+      r_bo[q][z] = in ? (gj - 1) * ipx + (ic - 1) : 0;
     }
   }
   // ---- black pairs: tile row ty (y = j0+ty), pair p = 1..TP ----
   bool b_on[NBLK], b_odd[NBLK], b_in[NBLK][2];
   T b_c1[NBLK][2], b_c2[NBLK][2], b_s4[NBLK][2], b_c3[NBLK], b_c4[NBLK];
   int b_row[NBLK], b_ow[NBLK][2], b_xc[NBLK][2], b_bo[NBLK][2], b_so[NBLK];
-  int b_xw[MODE == 2 ? NBLK : 1][2];        // MODE 2: x_in index of the west neighbour of b_xc
+  int b_xw[NORM ? NBLK : 1][2];             // MODE 2, 3: x_in index of the west neighbour of b_xc
+  constexpr bool USI = CALLER_IN || CALLER_OUT;
+  int b_si[USI ? NBLK : 1];                 // MODE 3, 4: the pair's first cell in the caller's layout
+  bool b_two[USI ? NBLK : 1];               // MODE 3, 4: the pair's second cell lies inside
 #pragma unroll
   for (int q = 0; q < NBLK; q++) {
     int e = min(t + q * NT, TY * TP - 1);
@@ -167,6 +206,10 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
     b_c4[q] = c4;
     b_row[q] = (ty + 1) * RP + p;           // red-ring row of tile row ty
     b_so[q] = on ? (gj - 1) * (int)L.px + (ia - 1) : 0;
+    if (USI) {
+      b_si[q] = on ? (gj - 1) * (int)U.px + (ia - 1) : 0;
+      b_two[q] = two;
+    }
 #pragma unroll
     for (int z = 0; z < 2; z++) {
       bool bfirst = ((gj + z) & 1) == 1;    // black is the first cell iff gj + k odd
@@ -179,8 +222,8 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
       b_s4[q][z] = c1 + c2 + c3 + c4;
       b_ow[q][z] = b_row[q] - (bfirst ? 1 : 0);   // red x neighbours: pairs p-1,p or p,p+1
       b_xc[q][z] = (bfirst ? 0 : SB) + (ty + 2) * RP + p;
-      b_bo[q][z] = in ? (gj - 1) * (int)L.px + (ic - 1) : 0;
-      if (MODE == 2) b_xw[q][z] = bfirst ? SB + (ty + 2) * RP + p - 1 : (ty + 2) * RP + p;
+      b_bo[q][z] = in ? (gj - 1) * ipx + (ic - 1) : 0;
+      if (NORM) b_xw[q][z] = bfirst ? SB + (ty + 2) * RP + p - 1 : (ty + 2) * RP + p;
     }
   }
   // ---- x_in staging: pair e of a plane (row ry, pair p) ----
@@ -197,7 +240,7 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
     // an odd gi <= nx keeps the 2-element load inside the padded row
     int gic = (gi >= 1 && gi <= L.nx) ? gi : 1;
     int gjc = min(max(gj, 1), L.ny);
-    x_off[q] = (gjc - 1) * (int)L.px + (gic - 1);
+    x_off[q] = (gjc - 1) * ipx + (gic - 1);
   }
 
   // Loads are issued unconditionally (clamped plane, precomputed in-bounds
@@ -207,9 +250,15 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   V2 xp[PF][NXL];
   T br[PF][NRED], bk[PF][NBLK];
   auto issue_x = [&](V2* dst, int m) {
-    const T* base = xi + (long)(min(max(m, 1), L.nz) - 1) * pxy;
+    const T* base = xi + (long)(min(max(m, 1), L.nz) - 1) * ipxy;
 #pragma unroll
-    for (int q = 0; q < NXL; q++) dst[q] = *reinterpret_cast<const V2*>(base + x_off[q]);
+    for (int q = 0; q < NXL; q++) {
+      if (CALLER_IN) {
+        dst[q] = ld_pair_unal(base + x_off[q]);
+      } else {
+        dst[q] = *reinterpret_cast<const V2*>(base + x_off[q]);
+      }
+    }
   };
   auto stage_x = [&](const V2* src, int m) {
     const unsigned ok = (m >= 1 && m <= L.nz) ? x_ok : 0u;
@@ -225,13 +274,13 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   };
   auto issue_bred = [&](T* dst, int m, auto Zc) {
     constexpr int Z = decltype(Zc)::value;
-    const T* base = b + (long)(min(max(m, 1), L.nz) - 1) * pxy;
+    const T* base = b + (long)(min(max(m, 1), L.nz) - 1) * ipxy;
 #pragma unroll
     for (int q = 0; q < NRED; q++) dst[q] = base[r_bo[q][Z]];
   };
   auto issue_bblk = [&](T* dst, int k, auto Zc) {
     constexpr int Z = decltype(Zc)::value;
-    const T* base = b + (long)(min(max(k, 1), L.nz) - 1) * pxy;
+    const T* base = b + (long)(min(max(k, 1), L.nz) - 1) * ipxy;
 #pragma unroll
     for (int q = 0; q < NBLK; q++) dst[q] = base[b_bo[q][Z]];
   };
@@ -287,6 +336,11 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
       // the pair is stored whole: a second cell beyond nx lies in the row's
       // padding (odd column <= nx, pitch a multiple of 16) and gets 0
       st_stream_pair(out + b_so[q], bf ? vb : vr, bf ? vr : vb);
+      if (CALLER_OUT) {   // the caller's row holds the ghost cell nx+1 past the last pair's first cell
+        T* uo = ext + (long)(k - 1) * U.pxy + b_si[q];
+        if (b_two[q]) st_pair_unal(uo, bf ? vb : vr, bf ? vr : vb);
+        else uo[0] = bf ? vb : vr;
+      }
     }
   };
 
@@ -294,16 +348,22 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   // cells of each pair: the black cell of parity Z is b_xc[q][Z], the red one
   // b_xc[q][1-Z] (the black cell of the other parity), each with its own
   // coefficients; b of the pair arrives as one aligned 2-element load.
-  V2 bn[MODE == 2 ? NBLK : 1];
+  V2 bn[NORM ? NBLK : 1];
   double acc = 0.0;
   auto issue_bn = [&](int m) {
-    if (MODE != 2) return;
-    const T* base = b + (long)(min(max(m, 1), L.nz) - 1) * pxy;
+    if (!NORM) return;
+    const T* base = b + (long)(min(max(m, 1), L.nz) - 1) * ipxy;
 #pragma unroll
-    for (int q = 0; q < NBLK; q++) bn[q] = *reinterpret_cast<const V2*>(base + b_so[q]);
+    for (int q = 0; q < NBLK; q++) {
+      if (CALLER_IN) {
+        bn[q] = ld_pair_unal(base + b_si[q]);
+      } else {
+        bn[q] = *reinterpret_cast<const V2*>(base + b_so[q]);
+      }
+    }
   };
   auto norm_plane = [&](int m, auto Zc) {
-    if (MODE != 2) return;
+    if (!NORM) return;
     constexpr int Z = decltype(Zc)::value;
     const T* X0 = sx + (m & 3) * PLANE;
     const T* Xm = sx + ((m - 1) & 3) * PLANE;
@@ -326,6 +386,12 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
         const T r = (first ? bn[q].x : bn[q].y) - ax;
         acc += (double)r * (double)r;
       }
+    }
+    if (CALLER_IN) {   // the tile's b into the solver's layout (a second cell beyond nx gets 0)
+      T* bo = ext + (long)(m - 1) * pxy;
+#pragma unroll
+      for (int q = 0; q < NBLK; q++)
+        if (b_on[q]) st_stream_pair(bo + b_so[q], bn[q].x, b_two[q] ? bn[q].y : T(0));
     }
   };
 
@@ -366,7 +432,7 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
     lds_barrier();
     red_plane(br[B], m, std::integral_constant<int, ZM>());     // needs b(m), x planes m-1..m+1
     issue_bred(br[B], m + PF, std::integral_constant<int, (ZM + PF) & 1>());
-    if (MODE == 2) {
+    if (NORM) {
       if (m >= k0 && m < k1) norm_plane(m, std::integral_constant<int, ZM>());
       issue_bn(m + 1);
     }
@@ -389,7 +455,8 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
 template <typename T, int TX, int TY, int MODE, int PF>
 __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, const T* __restrict__ xi,
                                                const T* __restrict__ b, const Lvl<T>& L, T omega,
-                                               const FusedGrid& G, double* __restrict__ partial) {
+                                               const FusedGrid& G, double* __restrict__ partial, const Lvl<T>& U,
+                                               T* __restrict__ ext) {
   using S = FusedShape<T, TX, TY>;
   // x_in planes (4 slots): first cells of the pairs (i = i0-2+2p, odd),
   // then second cells (i+1); new red values (4 slots), one per pair
@@ -402,8 +469,8 @@ __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, cons
   const int j0 = 1 + (rxy / G.gx) * TY;
   const int k0 = 1 + bz * G.kchunk;              // odd: kchunk is even
   const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
-  const double acc = rb_fused_body<T, TX, TY, MODE, PF>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr);
-  if (MODE == 2) {
+  const double acc = rb_fused_body<T, TX, TY, MODE, PF>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr, U, ext);
+  if (MODE == 2 || MODE == 3) {
     __shared__ double sh[fused_threads(TX, TY) / 64];
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) partial[bid] = s;
@@ -426,11 +493,15 @@ __device__ __forceinline__ int fused_xcd_bid(int w, int nb, const FusedGrid& G) 
   return (l - bz * nxy) * G.nch + bz;
 }
 
+// U, ext: the caller's layout and array of MODE 3 (b out in the solver's
+// layout is ext, x_in and b are the caller's) and MODE 4 (ext: the caller's
+// solution); unused otherwise.
 template <typename T, int TX, int TY, int MODE, int PF>
 __global__ void __launch_bounds__(fused_threads(TX, TY))
 k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega, FusedGrid G,
-           double* __restrict__ partial) {
-  rb_fused_block<T, TX, TY, MODE, PF>(fused_xcd_bid(blockIdx.x, gridDim.x, G), xo, xi, b, L, omega, G, partial);
+           double* __restrict__ partial, Lvl<T> U, T* __restrict__ ext) {
+  rb_fused_block<T, TX, TY, MODE, PF>(fused_xcd_bid(blockIdx.x, gridDim.x, G), xo, xi, b, L, omega, G, partial, U,
+                                      ext);
 }
 
 }  // namespace pepsmg

@@ -127,6 +127,23 @@ struct CgBufs {
   }
 };
 
+// Measurement hook: brackets the plain (mode 0) fused smoother launches on
+// one level, the finest level of the solve in progress, with HIP events on
+// the solve stream; the durations are summed once the solve has finished.
+// Launches beyond the event pool of one solve are not timed.
+struct KernelTimer {
+  bool on = false;
+  const void* level = nullptr;   // the DevLevel whose plain fused launches are timed
+  std::vector<hipEvent_t> ev;    // pairs (before, after)
+  int used = 0;                  // events recorded in the solve in progress
+  double total_ms = 0.0;
+  long launches = 0;
+  char name[96] = {0};           // the timed kernel's symbol
+  bool wants(const void* lvl, int mode) const {
+    return on && lvl == level && mode == 0 && used + 2 <= (int)ev.size();
+  }
+};
+
 }  // namespace pepsmg
 
 struct pepsmg_ctx {
@@ -156,6 +173,7 @@ struct pepsmg_ctx {
   hipEvent_t ev_norm = nullptr;
   hipEvent_t ev_entry = nullptr;   // orders the own stream after the caller's default-stream work
   bool sol_out_done = false;       // the last mixed-precision step stored x into the caller's array
+  pepsmg::KernelTimer timer;       // measurement hook (pepsmg_kernel_timing)
   // host copy of the ghost-cell flag of the solve in progress (read back with its first norm)
   int* ghost_flag_host() { return reinterpret_cast<int*>(h_scalars + 7); }
 

@@ -76,8 +76,7 @@ struct XStager {
       int gjc = min(max(gj, 1), L.ny);
       long q0 = lidx(L, gic, gjc, mc);
       if (UNAL) {   // rows of the caller's padded layout: pairs are 8-byte aligned
-        pre[q].x = x[q0];
-        pre[q].y = x[q0 + 1];
+        pre[q] = ld_pair_unal(x + q0);
       } else {
         pre[q] = *reinterpret_cast<const V2*>(x + q0);
       }
@@ -186,9 +185,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
     for (int q = 0; q < NP; q++) {
       int gi = act[q] ? i0 + 2 * lp[q] : 1, gj = act[q] ? j0 + ly[q] : 1;
       if (UNAL) {
-        const long o = lidx(LI, gi, gj, mc);
-        bpre[q].x = ld_stream(b + o);
-        bpre[q].y = ld_stream(b + o + 1);
+        bpre[q] = ld_pair_unal(b + lidx(LI, gi, gj, mc));
       } else {
         bpre[q] = ld_stream_v2(b + lidx(L, gi, gj, mc));
       }
@@ -266,8 +263,8 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
       }
       if (KIND == 6) {   // x_in into the caller's padded array (layout LI, base bout)
         const long u = lidx(LI, gi, gj, m);
-        bout[u] = s0[ly[q] + 1][2 * lp[q] + 2];
-        if (two) bout[u + 1] = s0[ly[q] + 1][2 * lp[q] + 3];
+        if (two) st_pair_unal(bout + u, s0[ly[q] + 1][2 * lp[q] + 2], s0[ly[q] + 1][2 * lp[q] + 3]);
+        else bout[u] = s0[ly[q] + 1][2 * lp[q] + 2];
       }
       if (KIND == 2) {
         if (two) {

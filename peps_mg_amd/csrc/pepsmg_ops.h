@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -103,13 +104,23 @@ struct Ops {
 
   // output buffer of the next fused launch (nullptr: d.x2)
   T* fused_out = nullptr;
+  // modes 3 and 4 (FP64): the caller's padded arrays (pointers at cell
+  // (1,1,1)) and their layout
+  const T* caller_x = nullptr;
+  const T* caller_b = nullptr;
+  T* caller_sol = nullptr;
+  Lvl<T> caller_lvl{};
 
   template <int TX, int TY, int MODE, int PF>
   void fused_launch(DevLevel<T>& d, T omega, double* partial) {
     FusedGrid G = fused_grid(d, TX, TY, fused_resident<TX, TY, PF>());
     unsigned nb = (unsigned)(G.gx * G.gy * G.nch);
+    const T* xi = MODE == 3 ? caller_x : d.x;
+    const T* b = MODE == 3 ? caller_b : d.b;
+    T* ext = MODE == 3 ? d.b : (MODE == 4 ? caller_sol : nullptr);
     hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(fused_threads(TX, TY)), 0, s,
-                       fused_out ? fused_out : d.x2, d.x, d.b, d.L(), omega, G, partial);
+                       fused_out ? fused_out : d.x2, xi, b, d.L(), omega, G, partial,
+                       (MODE == 3 || MODE == 4) ? caller_lvl : d.L(), ext);
   }
 
   template <int TX, int TY, int PF>
@@ -117,6 +128,8 @@ struct Ops {
     if (mode == 1) fused_launch<TX, TY, 1, PF>(d, omega, nullptr);
     else if constexpr (sizeof(T) == 8) {
       if (mode == 2) fused_launch<TX, TY, 2, PF>(d, omega, partial);
+      else if (mode == 3) fused_launch<TX, TY, 3, PF>(d, omega, partial);
+      else if (mode == 4) fused_launch<TX, TY, 4, PF>(d, omega, nullptr);
       else fused_launch<TX, TY, 0, PF>(d, omega, nullptr);
     } else {
       fused_launch<TX, TY, 0, PF>(d, omega, nullptr);
@@ -168,22 +181,58 @@ struct Ops {
 
   // One fused red+black iteration x -> x2, then swap.  mode 0 plain, 1 zero
   // start (x taken as 0), 2 (FP64) plain with the partial sums of
-  // (b - A x)^2 of the x it starts from written to partial[0 .. fused_blocks).
+  // (b - A x)^2 of the x it starts from written to partial[0 .. fused_blocks);
+  // FP64 on the caller's arrays (set_caller): 3 as 2 with x and b read from
+  // them and b stored into d.b, 4 as 0 with x also stored into the caller's sol.
   // prefetch: planes of look-ahead of the global loads.
   int prefetch = 1;
+  // the caller's padded (0:nx+1, 0:ny+1, 0:nz+1) arrays on level d, for modes 3 and 4
+  void set_caller(const DevLevel<T>& d, T* sol, const T* rhs) {
+    caller_lvl = d.L();
+    caller_lvl.px = d.nx + 2;
+    caller_lvl.pxy = (long)(d.nx + 2) * (d.ny + 2);
+    const long o = 1 + caller_lvl.px + caller_lvl.pxy;
+    caller_sol = sol + o;
+    caller_x = sol + o;
+    caller_b = rhs + o;
+  }
+
+  // measurement hook: plain launches on the timer's level are bracketed by events
+  KernelTimer* timer = nullptr;
+  void timer_begin(const DevLevel<T>& d, int mode, bool& timed) {
+    timed = timer && timer->wants(&d, mode);
+    if (timed) hipEventRecord(timer->ev[timer->used], s);
+  }
+  void timer_end(bool timed) {
+    if (!timed) return;
+    hipEventRecord(timer->ev[timer->used + 1], s);
+    timer->used += 2;
+  }
   void fused_iter(DevLevel<T>& d, T omega, int mode, double* partial = nullptr) {
     int ty = ty_used(d);
+    bool timed;
+    timer_begin(d, mode, timed);
     if (sizeof(T) == 4 && tx_used(d) == 64) fused_tx<64>(d, omega, mode, ty, partial);
     else fused_tx<FTX>(d, omega, mode, ty, partial);
+    timer_end(timed);
     std::swap(d.x, d.x2);
   }
   // one fused iteration (mode 0) from d.x into out; no swap
   void fused_iter_into(DevLevel<T>& d, T omega, T* out) {
     fused_out = out;
     int ty = ty_used(d);
+    bool timed;
+    timer_begin(d, 0, timed);
     if (sizeof(T) == 4 && tx_used(d) == 64) fused_tx<64>(d, omega, 0, ty, nullptr);
     else fused_tx<FTX>(d, omega, 0, ty, nullptr);
+    timer_end(timed);
     fused_out = nullptr;
+  }
+  // symbol of the plain fused kernel fused_iter launches on level d
+  void fused_name(const DevLevel<T>& d, char* buf, size_t n) const {
+    const int ty = ty_used(d);
+    std::snprintf(buf, n, "k_rb_fused<%s, %d, %d, 0, %d>", sizeof(T) == 8 ? "double" : "float", tx_used(d), ty,
+                  ty == 32 ? 1 : prefetch);   // 32-row tiles load one plane ahead
   }
 
   void smooth_colour(DevLevel<T>& d, const T* b, T* x, T omega, int colour, bool zero) {

@@ -41,20 +41,42 @@ __device__ __forceinline__ double cg_rows(const Lvl<T>& L, F&& f) {
   return acc;
 }
 
+// Whole pairs (i odd, i+1) of the solver's layout are 16-byte (FP64) or
+// 8-byte (FP32) aligned: rows start at multiples of the pitch (a multiple of
+// 16 elements) and a pair at an even offset.  The elementwise CG kernels
+// move them as one vector per array, so every store instruction writes
+// whole cache lines (two interleaved scalar stores wrote 1.24x the bytes of
+// x and r at 512^3); the arithmetic and the order of the sums are those of
+// the scalar form.
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type ld_pair(const T* p) {
+  return *reinterpret_cast<const typename Vec2<T>::type*>(p);
+}
+template <typename T>
+__device__ __forceinline__ void st_pair(T* p, T a, T b) {
+  typename Vec2<T>::type v;
+  v.x = a;
+  v.y = b;
+  *reinterpret_cast<typename Vec2<T>::type*>(p) = v;
+}
+
 // r = b - A 0 = b, p = r, partial (r, r) (x taken as 0, :196-204)
 template <typename T>
 __global__ void __launch_bounds__(64 * CG_WAVES)
 k_cg_start0(T* __restrict__ r, T* __restrict__ p, const T* __restrict__ b, Lvl<T> L, double* __restrict__ partial) {
   __shared__ double sh[CG_WAVES];
   double acc = cg_rows(L, [&](long q, bool two) {
-    T r0 = b[q] - T(0), r1 = two ? b[q + 1] - T(0) : T(0);
+    if (two) {
+      const auto bv = ld_pair(b + q);
+      const T r0 = bv.x - T(0), r1 = bv.y - T(0);
+      st_pair(r + q, r0, r1);
+      st_pair(p + q, r0, r1);
+      return (double)r0 * (double)r0 + (double)r1 * (double)r1;
+    }
+    const T r0 = b[q] - T(0);
     r[q] = r0;
     p[q] = r0;
-    if (two) {
-      r[q + 1] = r1;
-      p[q + 1] = r1;
-    }
-    return (double)r0 * (double)r0 + (double)r1 * (double)r1;
+    return (double)r0 * (double)r0 + (double)T(0) * (double)T(0);
   });
   double t = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -68,21 +90,28 @@ k_cg_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* 
   __shared__ double sh[CG_WAVES];
   const T alpha = (T)(sc[rr_slot] / sc[2]);
   double acc = cg_rows(L, [&](long q, bool two) {
-    T x0 = zero_x ? T(0) : x[q];
-    T n0 = x0 + alpha * p[q];
-    T s0 = r[q] - alpha * ap[q];
+    if (two) {
+      const auto pv = ld_pair(p + q), av = ld_pair(ap + q), rv = ld_pair(r + q);
+      T x0 = T(0), x1 = T(0);
+      if (!zero_x) {
+        const auto xv = ld_pair(x + q);
+        x0 = xv.x;
+        x1 = xv.y;
+      }
+      const T n0 = x0 + alpha * pv.x, n1 = x1 + alpha * pv.y;
+      const T s0 = rv.x - alpha * av.x, s1 = rv.y - alpha * av.y;
+      st_pair(x + q, n0, n1);
+      st_pair(r + q, s0, s1);
+      double a = (double)s0 * (double)s0;
+      a += (double)s1 * (double)s1;
+      return a;
+    }
+    const T x0 = zero_x ? T(0) : x[q];
+    const T n0 = x0 + alpha * p[q];
+    const T s0 = r[q] - alpha * ap[q];
     x[q] = n0;
     r[q] = s0;
-    double a = (double)s0 * (double)s0;
-    if (two) {
-      T x1 = zero_x ? T(0) : x[q + 1];
-      T n1 = x1 + alpha * p[q + 1];
-      T s1 = r[q + 1] - alpha * ap[q + 1];
-      x[q + 1] = n1;
-      r[q + 1] = s1;
-      a += (double)s1 * (double)s1;
-    }
-    return a;
+    return (double)s0 * (double)s0;
   });
   double t = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -94,8 +123,12 @@ __global__ void __launch_bounds__(64 * CG_WAVES)
 k_cg_p(T* __restrict__ p, const T* __restrict__ r, const double* __restrict__ sc, int new_slot, int old_slot, Lvl<T> L) {
   const T beta = (T)(sc[new_slot] / sc[old_slot]);
   cg_rows(L, [&](long q, bool two) {
-    p[q] = r[q] + beta * p[q];
-    if (two) p[q + 1] = r[q + 1] + beta * p[q + 1];
+    if (two) {
+      const auto rv = ld_pair(r + q), pv = ld_pair(p + q);
+      st_pair(p + q, rv.x + beta * pv.x, rv.y + beta * pv.y);
+    } else {
+      p[q] = r[q] + beta * p[q];
+    }
     return 0.0;
   });
 }
@@ -199,6 +232,9 @@ void pepsmg_ctx::release() {
   if (ev_norm) hipEventDestroy(ev_norm);
   if (ev_entry) hipEventDestroy(ev_entry);
   ev0 = ev1 = ev_norm = ev_entry = nullptr;
+  for (auto e : timer.ev)
+    if (e) hipEventDestroy(e);
+  timer.ev.clear();
   if (h_scalars) hipHostFree(h_scalars);
   h_scalars = nullptr;
   if (stream) hipStreamDestroy(stream);
@@ -207,13 +243,39 @@ void pepsmg_ctx::release() {
 
 namespace pepsmg {
 
+// Measurement hook: time the plain fused launches on level d of this solve.
+template <typename T>
+static void timer_attach(pepsmg_ctx* c, Ops<T>& op, DevLevel<T>& d) {
+  if (!c->timer.on) return;
+  op.timer = &c->timer;
+  c->timer.level = &d;
+  c->timer.used = 0;
+  op.fused_name(d, c->timer.name, sizeof c->timer.name);
+}
+
+// Adds the bracketed launches of the finished solve to the totals.
+static void timer_collect(pepsmg_ctx* c) {
+  KernelTimer& t = c->timer;
+  for (int q = 0; q + 1 < t.used; q += 2) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, t.ev[q], t.ev[q + 1]) == hipSuccess) {
+      t.total_ms += ms;
+      t.launches++;
+    }
+  }
+  t.used = 0;
+  t.level = nullptr;
+}
+
 // Smoothing step on level l of hierarchy h (rbgs_iterator or cg_iterator
 // on the level's x and b).  zero_start: x is 0 on entry (coarse levels on
 // the way down, src/multigrid.f90:1937).  prolong_first: first add the
 // prolongation of the next coarser level's x (src/multigrid.f90:1958-1959).
+// last_mode: fused-kernel mode of the last iteration (4: it also stores x
+// into the caller's array).
 template <typename T>
 static void smooth_level(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, int l, int nu, T omega,
-                         bool zero_start, bool prolong_first) {
+                         bool zero_start, bool prolong_first, int last_mode = 0) {
   DevLevel<T>& d = h.lv[l];
   bool fused = c->opt.fused_smoother && c->opt.smoother == 0 && nu > 0;
   if (prolong_first) op.prolong_add(d, h.lv[l + 1], d.x);
@@ -226,7 +288,8 @@ static void smooth_level(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>* cg, i
     op.rbgs(d, d.x, d.b, nu, omega, zero_start);
     return;
   }
-  for (int it = 0; it < nu; it++) op.fused_iter(d, omega, (it == 0 && zero_start) ? 1 : 0);
+  for (int it = 0; it < nu; it++)
+    op.fused_iter(d, omega, (it == 0 && zero_start) ? 1 : (it == nu - 1 && last_mode ? last_mode : 0));
 }
 
 // Coarsest level: FP64 solve in one workgroup (src/multigrid.f90:1944-1949,
@@ -317,15 +380,22 @@ static int diverged(int cyc, double rho) {
   return fail(PEPSMG_ERR_DIVERGED, "the relative residual became %g after %d cycle(s)", rho, cyc);
 }
 
-// FP64 V-cycles (src/multigrid.f90:1907-1976) on level-0 arrays already in
-// h64.lv[0].x / .b.
+// FP64 V-cycles (src/multigrid.f90:1907-1976).  ux, ub: the caller's
+// padded sol and rhs (device), read directly by the first smoothing
+// iteration (fused kernel mode 3), which stores b in the solver's layout;
+// the last iteration of a cycle expected to end the solve also stores x into
+// ux (mode 4), and c->sol_out_done records whether the solve ended on such a
+// cycle.  nullptr (or a path without the fused smoother's norm fold): x and
+// b are already in h64.lv[0].x / .b.
 static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_stats* st, std::vector<double>& hist,
-                      std::vector<int>& cit, double* res0_out, int* ncyc_out, double* rho_out) {
+                      std::vector<int>& cit, double* res0_out, int* ncyc_out, double* rho_out, double* ux = nullptr,
+                      const double* ub = nullptr) {
   Hier<double>& h = c->h64;
   Ops<double> op{c->stream};
   op.tile_y = c->opt.tile_y;
   op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
   op.tile_x = c->opt.tile_x;
+  timer_attach(c, op, h.lv[0]);
   CgBufs<double>* cg = nullptr;
   if (c->opt.smoother == 1) {
     cg = &c->cg64;
@@ -342,6 +412,10 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
   // separate pass is left only after the last permitted cycle.
   const bool fold = c->opt.fused_smoother && c->opt.smoother == 0 && nu > 0 && c->ncycle > 0;
   const int nbn = fold ? op.fused_blocks(f) : 0;
+  const bool direct = fold && ux != nullptr;
+  c->sol_out_done = false;
+  if (direct) op.set_caller(f, ux, ub);
+  int dual_cycle = -1;   // the cycle whose last iteration also stored x into ux
   double res0 = 0.0;
   int ncyc = 0;
   double rho = 1.0;
@@ -377,10 +451,23 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
     if (!std::isfinite(rho)) return diverged(cyc, rho);
     return rho < tol ? 1 : 0;
   };
+  // the cycle is predicted (the last two cycles' factor, no margin: a wrong
+  // prediction costs the extra stores, a missed one the copy-out pass, about
+  // the same) or bound (cycle cap) to end the solve; never before the first
+  // read-back, which brings the ghost flag of sol (nothing is stored into an
+  // array the entry point will reject)
+  auto ends_solve = [&](int cyc) {
+    if (!direct || cyc < 2 || *c->ghost_flag_host() != 0) return false;
+    if (cyc == c->ncycle) return true;
+    const size_t n = hist.size();
+    if (n < 2 || !(hist[n - 2] > 0.0)) return false;
+    return hist[n - 1] * (hist[n - 1] / hist[n - 2]) < tol;
+  };
   for (int cyc = first; cyc <= c->ncycle; cyc++) {
     if (fold) {
-      // first pre-smoothing iteration, with |b - A x|^2 of the incoming x
-      op.fused_iter(f, omega, 2, c->d_partial);
+      // first pre-smoothing iteration, with |b - A x|^2 of the incoming x (in
+      // cycle 1 of a direct solve from the caller's arrays)
+      op.fused_iter(f, omega, (cyc == 1 && direct) ? 3 : 2, c->d_partial);
       hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nbn,
                          c->d_scalars + (cyc == 1 ? 0 : 1));
       int pre_done = 1;
@@ -399,6 +486,7 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
         if (stop < 0) return stop;
         if (stop) {
           std::swap(f.x, f.x2);   // the solution is the iteration's input
+          c->sol_out_done = dual_cycle == cyc - 1;
           break;
         }
         if (ahead) {
@@ -413,8 +501,11 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
     op.residual_restrict(f, h.lv[1], f.x, f.b);                            // :1932-1933
     int* slot = (cyc - 1 < c->citers_cap) ? c->d_citers + (cyc - 1) : nullptr;
     vcycle_coarse_part<double>(c, op, h, cg, nu, nu, omega, tol, slot);    // :1936-1962
-    smooth_level(c, op, h, cg, 0, nu, omega, false, true);                 // :1965-1967
+    const bool dual = ends_solve(cyc);
+    if (dual) dual_cycle = cyc;
+    smooth_level(c, op, h, cg, 0, nu, omega, false, true, dual ? 4 : 0);  // :1965-1967
     if (!fold || cyc == c->ncycle) {
+      c->sol_out_done = dual_cycle == cyc;
       op.resid_norm_march(f, f.x, f.b, nullptr, c->d_partial, c->d_scalars + 1);  // :1970-1971
       HIP_TRY(hipMemcpyAsync(hs, c->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
@@ -451,6 +542,7 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   op.tile_y = c->opt.tile_y;
   op.prefetch = c->opt.fused_smoother == 2 ? 2 : 1;
   op.tile_x = c->opt.tile_x;
+  timer_attach(c, op, g.lv[0]);
   CgBufs<float>* cg = nullptr;
   if (c->opt.smoother == 1) {
     cg = &c->cg32;
@@ -654,11 +746,12 @@ static int run_solve(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_sta
   if (c->opt.precision == 1)
     rc = solve_mixed(c, nu, tol, omega, hist, cit, &res0, &ncyc, &nref, &rho, ux, ub);
   else
-    rc = solve_fp64(c, nu, tol, omega, st, hist, cit, &res0, &ncyc, &rho);
+    rc = solve_fp64(c, nu, tol, omega, st, hist, cit, &res0, &ncyc, &rho, ux, ub);
   if (rc && rc != PEPSMG_ERR_DIVERGED) return rc;   // (a diverged solve still reports its statistics)
   const int solve_rc = rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev1));
+  if (c->timer.on) timer_collect(c);
   HIP_TRY(hipGetLastError());   // a launch the runtime rejected leaves its error here
   if (st) {
     float ms = 0;
@@ -699,10 +792,12 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
   DevLevel<double>& f = c->h64.lv[0];
   int rc = PEPSMG_OK;
   int flag = 0;
-  // Mixed precision: the first refinement step reads the caller's arrays and
+  // Mixed precision, and FP64 with the fused smoother's norm fold: the first
+  // refinement step or smoothing iteration reads the caller's arrays and
   // leaves x and b in the solver's layout, so only the ghost layer of sol is
-  // checked here; FP64: copy-in passes (with the same check).
-  const bool direct = c->opt.precision == 1;
+  // checked here; other FP64 paths: copy-in passes (with the same check).
+  const bool direct = c->opt.precision == 1 || (c->opt.fused_smoother && c->opt.smoother == 0 && maxiteration > 0 &&
+                                                c->ncycle > 0);
   hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
   if (direct) {
     const long rows = (long)(f.ny + 2) * (f.nz + 2);
@@ -732,7 +827,8 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
     if (!(direct && c->sol_out_done)) unpack(c->stream, d_sol, f.x, f, 0);   // else the last step stored it
     if (d_rsd) {
       // a last refinement step stored x + e into sol only: take x from there
-      if (direct && c->sol_out_done) pack(c->stream, f.x, d_sol, f, (int*)nullptr);
+      // (the FP64 path stored it into both)
+      if (c->opt.precision == 1 && c->sol_out_done) pack(c->stream, f.x, d_sol, f, (int*)nullptr);
       Ops<double> op{c->stream};
       op.residual(f, f.r, f.x, f.b);                                   // :1970
       unpack(c->stream, d_rsd, f.r, f, 1);
@@ -819,6 +915,32 @@ int pepsmg_bench_smoother(pepsmg_ctx* c, int32_t precision, int32_t reps, double
   if (avg_ms) *avg_ms = ms / reps;
   if (bytes) *bytes = (fused ? 3.0 : 2.0) * n * (precision == 1 ? 4.0 : 8.0);
   if (kernel_name) *kernel_name = name;
+  return PEPSMG_OK;
+}
+
+int pepsmg_kernel_timing(pepsmg_ctx* c, int32_t enable) {
+  if (!c) return fail(PEPSMG_ERR_ARG, "NULL context");
+  KernelTimer& t = c->timer;
+  if (enable && t.ev.empty()) {
+    constexpr int POOL = 512;   // 256 timed launches per solve
+    t.ev.resize(POOL, nullptr);
+    for (auto& e : t.ev) HIP_TRY(hipEventCreate(&e));
+  }
+  t.on = enable != 0;
+  t.used = 0;
+  if (t.on) {   // switching off keeps the totals for pepsmg_kernel_timing_read
+    t.total_ms = 0.0;
+    t.launches = 0;
+    t.name[0] = 0;
+  }
+  return PEPSMG_OK;
+}
+
+int pepsmg_kernel_timing_read(const pepsmg_ctx* c, double* total_ms, int32_t* launches, const char** kernel_name) {
+  if (!c) return fail(PEPSMG_ERR_ARG, "NULL context");
+  if (total_ms) *total_ms = c->timer.total_ms;
+  if (launches) *launches = (int32_t)c->timer.launches;
+  if (kernel_name) *kernel_name = c->timer.name;
   return PEPSMG_OK;
 }
 

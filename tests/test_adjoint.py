@@ -120,3 +120,72 @@ def test_adjoint_dot_product_identity(stretch):
     rhs = (b * S.solve_adjoint(g)).sum().item()
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
     S.destroy()
+
+
+def inverse_source_problem(n, seed=20210501):
+    """The adjoint workload's data: f* = six Gaussians (centres U[0.2,0.8]^3,
+    widths U[0.03,0.1], amplitudes U[-1,1]), N(0, 1e-3) noise, a random
+    point f0 of U[-1,1]."""
+    rng = np.random.default_rng(seed)
+    c = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    fstar = np.zeros((n, n, n))
+    for _ in range(6):
+        cx, cy, cz = rng.uniform(0.2, 0.8, 3)
+        w = rng.uniform(0.03, 0.1)
+        fstar += rng.uniform(-1, 1) * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / (2 * w * w))
+    return fstar, rng.normal(scale=1e-3, size=(n, n, n)), rng.uniform(-1, 1, (n, n, n)), rng
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_adjoint_128_inverse_source(precision):
+    """The inverse-source workload at 128^3 (loss 0.5 |u(f) - u_obs|^2 +
+    1e-4 |grad f|^2, A u = -f): the gradient of the data term, which is the
+    adjoint solve's output, against a CPU FP64 reference built from the
+    oracle's multigrid solves (forward and adjoint, to 1e-13; A is symmetric
+    on this uniform grid); and central differences (eps 1e-6) of the whole
+    loss over 32 random unit directions.  The GPU solves run to 1e-12, which
+    leaves about 1e-11 relative error in u and in the gradient; the bound
+    is 1e-9."""
+    from oracle import oracle as O
+    from peps_mg_amd.adjoint import PoissonSolver, poisson_solve
+    n, levels = 128, 5
+    h = 1.0 / n
+    fstar, noise, f0, rng = inverse_source_problem(n)
+    g = O.Grid(n, n, n)
+    A = g.matrix()
+
+    def cpu_solve(rhs_interior):
+        b = O.padded(n, n, n)
+        b[1:-1, 1:-1, 1:-1] = rhs_interior
+        x = O.padded(n, n, n)
+        O.Multigrid(g, levels, 200).solve_vcycle(x, O.padded(n, n, n), A, b, 2, 1e-13, 1.0)
+        return x[1:-1, 1:-1, 1:-1].copy()
+
+    uobs = cpu_solve(-fstar) + noise
+    grad_ref = -cpu_solve(cpu_solve(-f0) - uobs)
+
+    S = PoissonSolver.uniform(n, levels, tolerance=1e-12, precision=precision)
+    uobs_t = torch.tensor(uobs, device="cuda")
+    f = torch.tensor(f0, device="cuda", requires_grad=True)
+    (0.5 * ((poisson_solve(-f, S) - uobs_t) ** 2).sum()).backward()
+    gd = f.grad.detach().cpu().numpy()
+    err = np.linalg.norm(gd - grad_ref) / np.linalg.norm(grad_ref)
+    assert err <= 1e-9, err
+
+    def loss(x):
+        reg = sum(((torch.diff(x, dim=d) / h) ** 2).sum() for d in range(3))
+        return 0.5 * ((poisson_solve(-x, S) - uobs_t) ** 2).sum() + 1e-4 * reg
+
+    f.grad = None
+    loss(f).backward()
+    gr = f.grad.detach()
+    eps = 1e-6
+    with torch.no_grad():
+        for _ in range(32):
+            d = torch.tensor(rng.normal(size=(n, n, n)), device="cuda")
+            d /= d.norm()
+            fd = (loss(f + eps * d) - loss(f - eps * d)).item() / (2 * eps)
+            assert abs(fd - (gr * d).sum().item()) <= 1e-5 * gr.norm().item()
+    S.destroy()

@@ -571,3 +571,62 @@ def test_solve_rejects_wrong_array_shapes():
         mg.solve_vcycle(good.copy(), good.copy(), matrix.MatrixHeptadiagonal(0, np.zeros((8, 12, 16, 6))), good,
                         2, 1e-10, 1.0)
     mg.destroy()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_kernel_timing_hook(precision):
+    """pepsmg_kernel_timing brackets the fine-level plain smoother launches
+    of the following solves with events; timed solves give the same result
+    bit for bit, and the kernel named is the one bench_smoother launches."""
+    import torch
+    n = 64
+    g = O.Grid(n, n, n)
+    mg = Multigrid(sdm_from(g), 4, 60, precision=precision)
+    b = torch.from_numpy(workload_rhs(g, 4)).cuda()
+    x0, x1 = torch.zeros_like(b), torch.zeros_like(b)
+    r0 = mg.solve_vcycle_device(x0.data_ptr(), 0, b.data_ptr(), 2, 1e-10, 1.0)
+    L = _lib.lib()
+    _lib.check(L.pepsmg_kernel_timing(mg.h, 1))
+    r1 = mg.solve_vcycle_device(x1.data_ptr(), 0, b.data_ptr(), 2, 1e-10, 1.0)
+    _lib.check(L.pepsmg_kernel_timing(mg.h, 0))
+    tot, cnt, name = C.c_double(0), C.c_int32(0), C.c_char_p()
+    _lib.check(L.pepsmg_kernel_timing_read(mg.h, C.byref(tot), C.byref(cnt), C.byref(name)))
+    timed_name = name.value
+    assert torch.equal(x0, x1) and r0.cycles == r1.cycles
+    # FP64 V(2,2): per cycle the pre-smoothing's second iteration and the
+    # post-smoothing's (the first pre-smoothing iteration forms the norm and
+    # the last one of the final cycle may store into sol: other modes)
+    assert cnt.value >= (2 * r1.cycles if precision == 0 else r1.cycles) and tot.value > 0
+    avg, nb, kn = C.c_double(0), C.c_double(0), C.c_char_p()
+    _lib.check(L.pepsmg_bench_smoother(mg.h, precision, 2, 1.0, C.byref(avg), C.byref(nb), C.byref(kn)))
+    assert timed_name == kn.value
+    mg.destroy()
+
+
+@pytest.mark.parametrize("nu", [2, 1])
+def test_fp64_caller_array_ends_bitwise(nu):
+    """FP64 solves on device arrays read sol and rhs in their first
+    smoothing iteration and store the last iteration of a cycle predicted
+    (or bound by the cycle cap) to end the solve straight into sol; over a
+    range of tolerances and caps the prediction is right, wrong, or never
+    made, and every sol and rsd equals the oracle's bit for bit (rsd
+    aliasing rhs as the entry point allows)."""
+    import torch
+    dims = (48, 40, 32)
+    g = O.Grid(*dims, l=(1.5, 1.25, 1.0))
+    A = g.matrix()
+    b = workload_rhs(g, 12)
+    for ncycle, tol in [(60, 1e-6), (60, 3e-8), (60, 1e-9), (60, 1e-10), (60, 1e-12), (4, 1e-12), (2, 1e-12)]:
+        mo = O.Multigrid(g, 3, ncycle)
+        xo, ro = O.padded(*dims), O.padded(*dims)
+        nc, hist, cit, res0 = mo.solve_vcycle(xo, ro, A, b, nu, tol, 1.0)
+        mg = Multigrid(sdm_from(g), 3, ncycle)
+        d_b = torch.from_numpy(b).cuda()
+        d_x = torch.zeros_like(d_b)
+        res = mg.solve_vcycle_device(d_x.data_ptr(), d_b.data_ptr(), d_b.data_ptr(), nu, tol, 1.0)
+        torch.cuda.synchronize()
+        assert res.cycles == nc, (ncycle, tol, res.cycles, nc)
+        assert np.max(np.abs(res.history - hist)) <= 1e-12
+        assert np.array_equal(d_x.cpu().numpy(), xo), (ncycle, tol)
+        assert np.array_equal(d_b.cpu().numpy(), ro), (ncycle, tol)
+        mg.destroy()
