@@ -630,3 +630,16 @@ def test_fp64_caller_array_ends_bitwise(nu):
         assert np.array_equal(d_x.cpu().numpy(), xo), (ncycle, tol)
         assert np.array_equal(d_b.cpu().numpy(), ro), (ncycle, tol)
         mg.destroy()
+
+
+@pytest.mark.parametrize("dims", [(64, 512, 32), (80, 528, 24)], ids=["whole-tiles", "partial-tiles"])
+def test_cg_smoother_tall_level(dims):
+    """A CG(4) V-cycle solve on a level of 512 rows or more, the second with
+    partial tiles in x and y, agrees with the oracle (cycle count, rho to
+    1e-12, x to 1e-10; the dot products are summed in another order)."""
+    nx, ny, nz = dims
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both(dims, (nx / 512, ny / 512, nz / 512), (1, 1, 1), 3, 4,
+                                                            1.0, smoother=1)
+    assert res.converged and res.cycles == nc
+    assert np.max(np.abs(res.history - hist)) <= 1e-12
+    assert np.abs(xg - xo).max() <= 1e-10 * np.abs(xo).max()
