@@ -816,7 +816,7 @@ def main():
             "traffic_read": round(tr["FETCH_SIZE"]) if tr else None,
             "traffic_write": round(tr["WRITE_SIZE"]) if tr else None,
             "kernel": kname64, "avg_ms": avg64, "launches": n64, "bytes_per_launch": nbytes64,
-            "timing": "HIP events on the solve stream around every launch of the kernel on the 512^3 level "
+            "timing": f"HIP events on the solve stream around every launch of the kernel on the {n}^3 level "
                       "inside the timed FP64 solves",
             "stream_ceiling_gbs": round(stream_gbs, 1), "frac_of_stream_ceiling": round(achieved / stream_gbs, 4),
             "back_to_back": {"avg_ms": bb64, "frac": frac(nbytes64, bb64),
@@ -824,7 +824,7 @@ def main():
                                      "(pepsmg_bench_smoother), events around the whole run"}}
     roof32 = {"achieved": round(achieved32, 1), "frac": round(achieved32 / HBM_PEAK_GBS, 4), "kernel": kname32,
               "avg_ms": avg32, "launches": n32, "bytes_per_launch": nbytes32,
-              "timing": "HIP events around every launch on the 512^3 level inside the timed mixed solves "
+              "timing": f"HIP events around every launch on the {n}^3 level inside the timed mixed solves "
                         "(default options)",
               "back_to_back": {"avg_ms": bb32, "frac": frac(nbytes32, bb32)}}
     if trace:
