@@ -340,7 +340,11 @@ k_refine_start(const double* __restrict__ x, const double* __restrict__ b, doubl
 // Residual + restriction for a level that halves every axis.  Fine tile
 // TX x 16 (TX = 64, or 128 for FP32: the same bytes per row), coarse tile
 // TX/2 x 8, one coarse cell per thread (4 TX threads); chunks hold an even
-// number of fine planes.
+// number of fine planes.  Each thread evaluates the residuals of its coarse
+// cell's own children: the fine cell pair under it on the two fine rows of
+// each of the two fine planes.  The four residuals of the first (odd) plane
+// wait in registers for the second, so the eight-term sum needs no LDS
+// round trip and no second barrier.
 template <int TX>
 constexpr int march_threads() { return 4 * TX; }
 
@@ -350,11 +354,9 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
                                                      const RestrictW<T>& W, const MarchGrid& G) {
   constexpr int TY = 16, TP = TX / 2, NT = march_threads<TX>();
   using St = XStager<T, TY, false, false, TX, NT>;
-  constexpr int RX = St::RX, RY = St::RY;
-  constexpr int NP = 2;   // pairs per thread: rows ly and ly + 8
+  constexpr int RX = St::RX;
   using V2 = typename Vec2<T>::type;
-  __shared__ T sx[4][RY][RX];
-  __shared__ T sres[2][TY][TX];
+  __shared__ T sx[4][St::RY][RX];
 
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
@@ -363,44 +365,31 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
   const int k0 = 1 + bz * G.kchunk;                 // odd
   const int k1 = min(k0 + G.kchunk, L.nz + 1);
 
+  // coarse cell of this thread: (ic, jc) = (ic0 + t%TP, jc0 + t/TP); its
+  // children are fine cells iz = 2ic-1, ip = 2ic (tile columns 2 lic and
+  // 2 lic + 1) on rows jz = 2jc-1, jp = 2jc (tile rows 2 ljc and 2 ljc + 1)
   const int t = threadIdx.x;
-  int ly[NP], lp[NP];
-  bool act[NP];
-  T c1a[NP], c2a[NP], c1b[NP], c2b[NP], c3[NP], c4[NP];
-#pragma unroll
-  for (int q = 0; q < NP; q++) {
-    ly[q] = t / TP + 8 * q;
-    lp[q] = t % TP;
-    int gi = i0 + 2 * lp[q], gj = j0 + ly[q];
-    act[q] = gj <= L.ny && gi <= L.nx;
-    c1a[q] = act[q] ? L.cwx[gi] : T(0);
-    c2a[q] = act[q] ? L.cex[gi] : T(0);
-    c1b[q] = (act[q] && gi + 1 <= L.nx) ? L.cwx[gi + 1] : T(0);
-    c2b[q] = (act[q] && gi + 1 <= L.nx) ? L.cex[gi + 1] : T(0);
-    c3[q] = act[q] ? L.cwy[gj] : T(0);
-    c4[q] = act[q] ? L.cey[gj] : T(0);
-  }
-  // coarse cell of this thread: (ic, jc) = (ic0 + t%TP, jc0 + t/TP)
   const int lic = t % TP, ljc = t / TP;
   const int ic = (i0 + 1) / 2 + lic, jc = (j0 + 1) / 2 + ljc;
   const bool cact = ic <= Cc.nx && jc <= Cc.ny;
   const int iz = 2 * ic - 1, ip = 2 * ic, jz = 2 * jc - 1, jp = 2 * jc;
+  const int lxz = 2 * lic, lyz = 2 * ljc;
+  T c1a = T(0), c2a = T(0), c1b = T(0), c2b = T(0), c3z = T(0), c4z = T(0), c3p = T(0), c4p = T(0);
   T ax1 = T(0), ax2 = T(0), ay1 = T(0), ay2 = T(0);
-  if (cact) {
+  if (cact) {   // a halving level: every child of an active coarse cell is in the domain
+    c1a = L.cwx[iz]; c2a = L.cex[iz]; c1b = L.cwx[ip]; c2b = L.cex[ip];
+    c3z = L.cwy[jz]; c4z = L.cey[jz]; c3p = L.cwy[jp]; c4p = L.cey[jp];
     ax1 = W.wx[iz]; ax2 = W.wx[ip]; ay1 = W.wy[jz]; ay2 = W.wy[jp];
   }
 
   St st;
-  V2 bpre[NP];
-  // b of this thread's pairs, from clamped addresses (values of cells
-  // outside the domain are never used)
+  V2 bpre[2];   // b of the children pairs on rows jz and jp
+  // from clamped addresses (values of cells outside the domain are never used)
   auto issue_b = [&](int m) {
     const int mc = min(max(m, 1), L.nz);
-#pragma unroll
-    for (int q = 0; q < NP; q++) {
-      int gi = act[q] ? i0 + 2 * lp[q] : 1, gj = act[q] ? j0 + ly[q] : 1;
-      bpre[q] = ld_stream_v2(b + lidx(L, gi, gj, mc));
-    }
+    const int gi = cact ? iz : 1, g0 = cact ? jz : 1, g1 = cact ? jp : 1;
+    bpre[0] = ld_stream_v2(b + lidx(L, gi, g0, mc));
+    bpre[1] = ld_stream_v2(b + lidx(L, gi, g1, mc));
   };
 
   st.issue(x, nullptr, L, i0, j0, k0 - 1);
@@ -409,53 +398,46 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
   st.stage(sx[k0 & 3], L, i0, j0);
   st.issue(x, nullptr, L, i0, j0, k0 + 1);
   issue_b(k0);
+  // residuals of the odd plane kz = 2kc-1: (x child, y child) = (z,z), (p,z), (z,p), (p,p)
+  T rzz = T(0), rpz = T(0), rzp = T(0), rpp = T(0);
   // plane step, run twice per iteration (chunks hold an even number of
-  // planes) so the compiler's wait counts across the back edge stay exact
-  auto step = [&](int m) {
+  // planes, starting on an odd one) so the compiler's wait counts across
+  // the back edge stay exact and each step knows its plane's parity
+  auto step = [&](int m, auto odd_t) {
+    constexpr bool ODD = decltype(odd_t)::value;
     st.stage(sx[(m + 1) & 3], L, i0, j0);
     lds_barrier();
     T(*s0)[RX] = sx[m & 3];
     T(*sm)[RX] = sx[(m - 1) & 3];
     T(*sp)[RX] = sx[(m + 1) & 3];
     T c5 = ldc(L.cwz, m), c6 = ldc(L.cez, m);
-    V2 bv[NP];
-#pragma unroll
-    for (int q = 0; q < NP; q++) bv[q] = bpre[q];
+    V2 bz0 = bpre[0], bz1 = bpre[1];
     st.issue(x, nullptr, L, i0, j0, m + 2);
     issue_b(m + 1);
-    T(*rr)[TX] = sres[(m + 1) & 1];   // fine plane 2kc-1 -> slot 0, 2kc -> slot 1
-#pragma unroll
-    for (int q = 0; q < NP; q++) {
-      T ra = T(0), rb = T(0);
-      if (act[q]) {
-        int gi = i0 + 2 * lp[q];
-        ra = march_resid<T, RX>(s0, sm, sp, 2 * lp[q], ly[q], c1a[q], c2a[q], c3[q], c4[q], c5, c6, bv[q].x);
-        if (gi + 1 <= L.nx)
-          rb = march_resid<T, RX>(s0, sm, sp, 2 * lp[q] + 1, ly[q], c1b[q], c2b[q], c3[q], c4[q], c5, c6, bv[q].y);
-      }
-      rr[ly[q]][2 * lp[q]] = ra;
-      rr[ly[q]][2 * lp[q] + 1] = rb;
+    T qzz = T(0), qpz = T(0), qzp = T(0), qpp = T(0);
+    if (cact) {
+      qzz = march_resid<T, RX>(s0, sm, sp, lxz, lyz, c1a, c2a, c3z, c4z, c5, c6, bz0.x);
+      qpz = march_resid<T, RX>(s0, sm, sp, lxz + 1, lyz, c1b, c2b, c3z, c4z, c5, c6, bz0.y);
+      qzp = march_resid<T, RX>(s0, sm, sp, lxz, lyz + 1, c1a, c2a, c3p, c4p, c5, c6, bz1.x);
+      qpp = march_resid<T, RX>(s0, sm, sp, lxz + 1, lyz + 1, c1b, c2b, c3p, c4p, c5, c6, bz1.y);
     }
-    if ((m & 1) == 0) {
-      lds_barrier();
-      int kc = m / 2;
-      if (cact) {
-        T az1 = ldc(W.wz, m - 1), az2 = ldc(W.wz, m);
-        T v111 = ax1 * ay1 * az1, v211 = ax2 * ay1 * az1, v121 = ax1 * ay2 * az1, v221 = ax2 * ay2 * az1;
-        T v112 = ax1 * ay1 * az2, v212 = ax2 * ay1 * az2, v122 = ax1 * ay2 * az2, v222 = ax2 * ay2 * az2;
-        T vol_c = v111 + v211 + v121 + v221 + v112 + v212 + v122 + v222;
-        int lxz = 2 * lic, lxp = 2 * lic + 1, lyz = 2 * ljc, lyp = 2 * ljc + 1;
-        // slot 1 = fine plane kp = 2kc, slot 0 = kz = 2kc-1
-        T s = v111 * sres[1][lyp][lxp] + v211 * sres[1][lyp][lxz] + v121 * sres[1][lyz][lxp] +
-              v221 * sres[1][lyz][lxz] + v112 * sres[0][lyp][lxp] + v212 * sres[0][lyp][lxz] +
-              v122 * sres[0][lyz][lxp] + v222 * sres[0][lyz][lxz];
-        __builtin_nontemporal_store(s / vol_c, bc + lidx(Cc, ic, jc, kc));
-      }
+    if (ODD) {
+      rzz = qzz; rpz = qpz; rzp = qzp; rpp = qpp;
+    } else if (cact) {
+      // (src/multigrid.f90:1460-1494) the even plane kp = 2kc's four terms
+      // first, then the odd plane's, each in the order (p,p), (z,p), (p,z), (z,z)
+      const int kc = m / 2;
+      T az1 = ldc(W.wz, m - 1), az2 = ldc(W.wz, m);
+      T v111 = ax1 * ay1 * az1, v211 = ax2 * ay1 * az1, v121 = ax1 * ay2 * az1, v221 = ax2 * ay2 * az1;
+      T v112 = ax1 * ay1 * az2, v212 = ax2 * ay1 * az2, v122 = ax1 * ay2 * az2, v222 = ax2 * ay2 * az2;
+      T vol_c = v111 + v211 + v121 + v221 + v112 + v212 + v122 + v222;
+      T s = v111 * qpp + v211 * qzp + v121 * qpz + v221 * qzz + v112 * rpp + v212 * rzp + v122 * rpz + v222 * rzz;
+      __builtin_nontemporal_store(s / vol_c, bc + lidx(Cc, ic, jc, kc));
     }
   };
   for (int m = k0; m < k1; m += 2) {
-    step(m);
-    step(m + 1);
+    step(m, std::true_type{});
+    step(m + 1, std::false_type{});
   }
 }
 
