@@ -128,7 +128,7 @@ kb_resid(const ResidItem* __restrict__ items, int n, const int* __restrict__ act
   const ResidItem& it = items[q];
   if (!active[it.item]) return;
   resid_march_block<double, RESID_TY, KIND>(blockIdx.x - it.block0, it.x, it.e, it.b, (double*)nullptr, it.r32, it.xnew,
-                                     it.L, it.G, it.partial, it.L, (double*)nullptr);
+                                     it.L, it.G, it.partial, it.L, it.L, (double*)nullptr, it.L);
 }
 
 template <typename T>

@@ -172,7 +172,7 @@ struct pepsmg_ctx {
   double* h_scalars = nullptr;
   hipEvent_t ev_norm = nullptr;
   hipEvent_t ev_entry = nullptr;   // orders the own stream after the caller's default-stream work
-  bool sol_out_done = false;       // the last mixed-precision step stored x into the caller's array
+  bool sol_out_done = false;       // the caller's sol holds the solution (stored there by the last step, or never left it)
   pepsmg::KernelTimer timer;       // measurement hook (pepsmg_kernel_timing)
   // host copy of the ghost-cell flag of the solve in progress (read back with its first norm)
   int* ghost_flag_host() { return reinterpret_cast<int*>(h_scalars + 7); }

@@ -14,7 +14,9 @@
 //                          src/multigrid.f90:1970-1971)
 //   k_resid_march<T,TY,1>: same, and stores r
 //   k_resid_march<double,TY,2>: iterative-refinement step: x_in = x + e
-//       (e the FP32 correction), stores x_in, r as float, partial r^2
+//       (e the FP32 correction), stores x_in, r as float, partial r^2; x
+//       and b in layouts of their own (the solver's, or the caller's padded
+//       arrays, whose pairs are only 8-byte aligned)
 //   k_cg_ap_fold<T,TY> (KIND 7): KIND 3 on pnew = r + beta p, formed while
 //                          staging; pnew stored too (the direction update
 //                          of the previous CG iteration, folded in)
@@ -26,9 +28,9 @@
 //       x_in = x + e stored into the caller's padded array only (no copy-out
 //       pass, no solver-layout x, no r32) and partial r^2
 //   k_refine_start:        first iterative-refinement step read straight
-//       from the caller's padded (0:n+1)^3 arrays (unaligned rows): stores x
-//       and b in the solver's layout, r as float and partial r^2 (the mixed
-//       solve needs no separate copy-in pass)
+//       from the caller's padded (0:n+1)^3 arrays (unaligned rows): stores
+//       only r as float and partial r^2 (x stays in the caller's array until
+//       the first update, b for the whole solve)
 //   k_restrict_march<T>:   residual + restriction into the next level's b
 //       (src/multigrid.f90:1932-1933, :1939-1940 with :1460-1494), for
 //       levels that halve every axis
@@ -62,26 +64,27 @@ struct XStager {
   T beta = T(0);
   int mq;
 
-  __device__ __forceinline__ void issue(const T* __restrict__ x, const float* __restrict__ e, const Lvl<T>& L, int i0,
-                                        int j0, int m) {
+  // LX: layout of x; LE: layout of e and cgp
+  __device__ __forceinline__ void issue(const T* __restrict__ x, const Lvl<T>& LX, const float* __restrict__ e,
+                                        const Lvl<T>& LE, int i0, int j0, int m) {
     mq = m;
-    const int mc = min(max(m, 1), L.nz);
+    const int mc = min(max(m, 1), LE.nz);
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
       int idx = min((int)threadIdx.x + q * NT, RY * XP - 1);
       int ry = idx / XP, px = idx - ry * XP;
       int gi = i0 - 2 + 2 * px, gj = j0 - 1 + ry;      // gi is odd
       // an odd gi <= nx keeps the 2-element load inside the padded row
-      int gic = (gi >= 1 && gi <= L.nx) ? gi : 1;
-      int gjc = min(max(gj, 1), L.ny);
-      long q0 = lidx(L, gic, gjc, mc);
-      if (UNAL) {   // rows of the caller's padded layout: pairs are 8-byte aligned
-        pre[q] = ld_pair_unal(x + q0);
+      int gic = (gi >= 1 && gi <= LE.nx) ? gi : 1;
+      int gjc = min(max(gj, 1), LE.ny);
+      if (UNAL) {   // x in a layout of its own, pairs possibly only 8-byte aligned
+        pre[q] = ld_pair_unal(x + lidx(LX, gic, gjc, mc));
       } else {
-        pre[q] = *reinterpret_cast<const V2*>(x + q0);
+        pre[q] = *reinterpret_cast<const V2*>(x + lidx(LX, gic, gjc, mc));
       }
-      if (MIXED) epre[q] = *reinterpret_cast<const float2*>(e + q0);
-      if (CGP) ppre[q] = *reinterpret_cast<const V2*>(cgp + q0);
+      const long qe = lidx(LE, gic, gjc, mc);
+      if (MIXED) epre[q] = *reinterpret_cast<const float2*>(e + qe);
+      if (CGP) ppre[q] = *reinterpret_cast<const V2*>(cgp + qe);
     }
   }
   __device__ __forceinline__ void stage(T (*s)[RX], const Lvl<T>& L, int i0, int j0) {
@@ -129,16 +132,21 @@ __device__ __forceinline__ T march_resid(T (*s0)[RX], T (*sm)[RX], T (*sp)[RX], 
   return bv - march_apply<T, RX>(s0, sm, sp, lx, ly, c1, c2, c3, c4, c5, c6);
 }
 
-// LI: layout of the inputs x and b (L itself except for KIND 5, where it
-// describes the caller's padded arrays); bout: KIND 5's copy of b.
+// L: the solver's layout (tile geometry; e, rout, r32, xnew).  LX, LB:
+// layouts of the inputs x and b for the refinement kinds 2, 5 and 6 (the
+// solver's, or the caller's padded arrays, read as 8-byte-aligned pairs);
+// the other kinds read both in L.  uout, LU: KIND 6's output, the caller's
+// padded array.
 template <typename T, int TY, int KIND>
 __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__ x, const float* __restrict__ e,
                                                   const T* __restrict__ b, T* __restrict__ rout,
                                                   float* __restrict__ r32, T* __restrict__ xnew, const Lvl<T>& L,
                                                   const MarchGrid& G, double* __restrict__ partial,
-                                                  const Lvl<T>& LI, T* __restrict__ bout, T beta = T(0)) {
+                                                  const Lvl<T>& LX, const Lvl<T>& LB, T* __restrict__ uout,
+                                                  const Lvl<T>& LU, T beta = T(0)) {
   constexpr bool MIXED = (KIND == 2 || KIND == 6);
-  constexpr bool UNAL = (KIND == 5);
+  constexpr bool UNAL = (KIND == 2 || KIND == 5);            // x in layout LX
+  constexpr bool BU = (KIND == 2 || KIND == 5 || KIND == 6);  // b in layout LB
   constexpr bool CGP = (KIND == 7);   // x is r, b the previous direction p
   constexpr bool AP = (KIND == 3 || KIND == 7);
   using St = XStager<T, TY, MIXED, UNAL, 64, MARCH_THREADS, CGP>;
@@ -184,24 +192,24 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
 #pragma unroll
     for (int q = 0; q < NP; q++) {
       int gi = act[q] ? i0 + 2 * lp[q] : 1, gj = act[q] ? j0 + ly[q] : 1;
-      if (UNAL) {
-        bpre[q] = ld_pair_unal(b + lidx(LI, gi, gj, mc));
+      if (BU) {
+        bpre[q] = ld_pair_unal(b + lidx(LB, gi, gj, mc));
       } else {
         bpre[q] = ld_stream_v2(b + lidx(L, gi, gj, mc));
       }
     }
   };
 
-  const Lvl<T>& LX = UNAL ? LI : L;   // layout of the x input
+  const Lvl<T>& LXI = UNAL ? LX : L;   // layout of the x input
   if (CGP) {
     st.cgp = b;
     st.beta = beta;
   }
-  st.issue(x, e, LX, i0, j0, k0 - 1);
+  st.issue(x, LXI, e, L, i0, j0, k0 - 1);
   st.stage(sx[(k0 - 1) & 3], L, i0, j0);
-  st.issue(x, e, LX, i0, j0, k0);
+  st.issue(x, LXI, e, L, i0, j0, k0);
   st.stage(sx[k0 & 3], L, i0, j0);
-  st.issue(x, e, LX, i0, j0, k0 + 1);
+  st.issue(x, LXI, e, L, i0, j0, k0 + 1);
   issue_b(k0);
   double acc = 0.0;
   // plane step; the loop below runs it twice per iteration so that the
@@ -217,7 +225,7 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
     V2 bv[NP];
 #pragma unroll
     for (int q = 0; q < NP; q++) bv[q] = bpre[q];
-    st.issue(x, e, LX, i0, j0, m + 2);
+    st.issue(x, LXI, e, L, i0, j0, m + 2);
     issue_b(m + 1);
 #pragma unroll
     for (int q = 0; q < NP; q++) {
@@ -250,21 +258,14 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
         if (two) st_stream_pair(xnew + q0, ra, rb);
         else xnew[q0] = ra;
       }
-      if (KIND == 5) {   // x and b into the solver's layout
-        if (two) {
-          st_stream_pair(xnew + q0, s0[ly[q] + 1][2 * lp[q] + 2], s0[ly[q] + 1][2 * lp[q] + 3]);
-          st_stream_pair(bout + q0, bv[q].x, bv[q].y);
-          st_stream_pair(r32 + q0, (float)ra, (float)rb);
-        } else {
-          xnew[q0] = s0[ly[q] + 1][2 * lp[q] + 2];
-          bout[q0] = bv[q].x;
-          r32[q0] = (float)ra;
-        }
+      if (KIND == 5) {   // the residual only: x and b stay in the caller's arrays
+        if (two) st_stream_pair(r32 + q0, (float)ra, (float)rb);
+        else r32[q0] = (float)ra;
       }
-      if (KIND == 6) {   // x_in into the caller's padded array (layout LI, base bout)
-        const long u = lidx(LI, gi, gj, m);
-        if (two) st_pair_unal(bout + u, s0[ly[q] + 1][2 * lp[q] + 2], s0[ly[q] + 1][2 * lp[q] + 3]);
-        else bout[u] = s0[ly[q] + 1][2 * lp[q] + 2];
+      if (KIND == 6) {   // x_in into the caller's padded array (layout LU)
+        const long u = lidx(LU, gi, gj, m);
+        if (two) st_pair_unal(uout + u, s0[ly[q] + 1][2 * lp[q] + 2], s0[ly[q] + 1][2 * lp[q] + 3]);
+        else uout[u] = s0[ly[q] + 1][2 * lp[q] + 2];
       }
       if (KIND == 2) {
         if (two) {
@@ -297,7 +298,19 @@ template <typename T, int TY, int KIND>
 __global__ void __launch_bounds__(MARCH_THREADS)
 k_resid_march(const T* __restrict__ x, const float* __restrict__ e, const T* __restrict__ b, T* __restrict__ rout,
               float* __restrict__ r32, T* __restrict__ xnew, Lvl<T> L, MarchGrid G, double* __restrict__ partial) {
-  resid_march_block<T, TY, KIND>(blockIdx.x, x, e, b, rout, r32, xnew, L, G, partial, L, (T*)nullptr);
+  resid_march_block<T, TY, KIND>(blockIdx.x, x, e, b, rout, r32, xnew, L, G, partial, L, L, (T*)nullptr, L);
+}
+
+// Refinement step (KIND 2): x + e into xnew (layout L) and r32, partial sums
+// of r^2; x in layout LX and b in layout LB (the solver's, or the caller's
+// padded arrays: pitches nx+2 and (nx+2)(ny+2), base at cell (1,1,1)).
+template <int TY>
+__global__ void __launch_bounds__(MARCH_THREADS)
+k_refine_step(const double* __restrict__ x, const float* __restrict__ e, const double* __restrict__ b,
+              float* __restrict__ r32, double* __restrict__ xnew, Lvl<double> L, Lvl<double> LX, Lvl<double> LB,
+              MarchGrid G, double* __restrict__ partial) {
+  resid_march_block<double, TY, 2>(blockIdx.x, x, e, b, (double*)nullptr, r32, xnew, L, G, partial, LX, LB,
+                                   (double*)nullptr, L);
 }
 
 // CG step with the direction update folded in (KIND 7): pnew = r + beta p
@@ -310,31 +323,30 @@ __global__ void __launch_bounds__(MARCH_THREADS)
 k_cg_ap_fold(const T* __restrict__ r, const T* __restrict__ p, T* __restrict__ pnew, T* __restrict__ ap, Lvl<T> L,
              MarchGrid G, double* __restrict__ partial, const double* __restrict__ sc, int new_slot, int old_slot) {
   const T beta = (T)(sc[new_slot] / sc[old_slot]);
-  resid_march_block<T, TY, 7>(blockIdx.x, r, (const float*)nullptr, p, ap, (float*)nullptr, pnew, L, G, partial, L,
-                              (T*)nullptr, beta);
+  resid_march_block<T, TY, 7>(blockIdx.x, r, (const float*)nullptr, p, ap, (float*)nullptr, pnew, L, G, partial, L, L,
+                              (T*)nullptr, L, beta);
 }
 
 // Refinement step expected to be the last (KIND 6): x_in = x + e goes only
 // into the caller's padded array usr (layout U: pitches nx+2 and
-// (nx+2)(ny+2), base at cell (1,1,1)), with the partial sums of r^2.  xnew
-// and r32 are not written.
+// (nx+2)(ny+2), base at cell (1,1,1)), with the partial sums of r^2; x and e
+// in the solver's layout L, b in layout LB.  No r32, no solver-layout x.
 template <int TY>
 __global__ void __launch_bounds__(MARCH_THREADS)
-k_refine_last(const double* __restrict__ x, const float* __restrict__ e, const double* __restrict__ b,
-              float* __restrict__ r32, double* __restrict__ xnew, Lvl<double> L, Lvl<double> U, MarchGrid G,
-              double* __restrict__ partial, double* __restrict__ usr) {
-  resid_march_block<double, TY, 6>(blockIdx.x, x, e, b, (double*)nullptr, r32, xnew, L, G, partial, U, usr);
+k_refine_last(const double* __restrict__ x, const float* __restrict__ e, const double* __restrict__ b, Lvl<double> L,
+              Lvl<double> LB, Lvl<double> U, MarchGrid G, double* __restrict__ partial, double* __restrict__ usr) {
+  resid_march_block<double, TY, 6>(blockIdx.x, x, e, b, (double*)nullptr, (float*)nullptr, (double*)nullptr, L, G,
+                                   partial, L, LB, usr, U);
 }
 
-// First refinement step from the caller's arrays: U is the layout of x and b
-// (pitches nx+2 and (nx+2)(ny+2), base at cell (1,1,1)).
+// First refinement step from the caller's arrays (layout U of x and b):
+// r32 = b - A x and the partial sums of r^2, nothing else.
 template <int TY>
 __global__ void __launch_bounds__(MARCH_THREADS)
-k_refine_start(const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ xout,
-               double* __restrict__ bout, float* __restrict__ r32, Lvl<double> L, Lvl<double> U, MarchGrid G,
-               double* __restrict__ partial) {
-  resid_march_block<double, TY, 5>(blockIdx.x, x, (const float*)nullptr, b, (double*)nullptr, r32, xout, L, G, partial,
-                                   U, bout);
+k_refine_start(const double* __restrict__ x, const double* __restrict__ b, float* __restrict__ r32, Lvl<double> L,
+               Lvl<double> U, MarchGrid G, double* __restrict__ partial) {
+  resid_march_block<double, TY, 5>(blockIdx.x, x, (const float*)nullptr, b, (double*)nullptr, r32, (double*)nullptr, L,
+                                   G, partial, U, U, (double*)nullptr, U);
 }
 
 // Residual + restriction for a level that halves every axis.  Fine tile
@@ -392,11 +404,11 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
     bpre[1] = ld_stream_v2(b + lidx(L, gi, g1, mc));
   };
 
-  st.issue(x, nullptr, L, i0, j0, k0 - 1);
+  st.issue(x, L, nullptr, L, i0, j0, k0 - 1);
   st.stage(sx[(k0 - 1) & 3], L, i0, j0);
-  st.issue(x, nullptr, L, i0, j0, k0);
+  st.issue(x, L, nullptr, L, i0, j0, k0);
   st.stage(sx[k0 & 3], L, i0, j0);
-  st.issue(x, nullptr, L, i0, j0, k0 + 1);
+  st.issue(x, L, nullptr, L, i0, j0, k0 + 1);
   issue_b(k0);
   // residuals of the odd plane kz = 2kc-1: (x child, y child) = (z,z), (p,z), (z,p), (p,p)
   T rzz = T(0), rpz = T(0), rzp = T(0), rpp = T(0);
@@ -412,7 +424,7 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
     T(*sp)[RX] = sx[(m + 1) & 3];
     T c5 = ldc(L.cwz, m), c6 = ldc(L.cez, m);
     V2 bz0 = bpre[0], bz1 = bpre[1];
-    st.issue(x, nullptr, L, i0, j0, m + 2);
+    st.issue(x, L, nullptr, L, i0, j0, m + 2);
     issue_b(m + 1);
     T qzz = T(0), qpz = T(0), qzp = T(0), qpp = T(0);
     if (cact) {

@@ -529,10 +529,12 @@ static int solve_fp64(pepsmg_ctx* c, int nu, double tol, double omega, pepsmg_st
 
 // FP32 V-cycles inside FP64 iterative refinement.  The fine residual
 // b - A x, the update x += e and the coarsest solve stay in FP64.
-// ux, ub: the caller's padded x and b (device), read directly by the first
-// refinement step; nullptr: x and b are already in the solver's layout.
-// The step predicted to be the last also stores its x into ux, and
-// c->sol_out_done records whether the solve ended on such a step.
+// ux, ub: the caller's padded x and b (device).  b is read from there by
+// every refinement step; x until the first update, which stores x + e in
+// the solver's layout.  The step predicted to be the last stores its x
+// into ux.  c->sol_out_done records whether ux holds the solution when the
+// solve ends (no update at all, or a last step that stored into it).
+// nullptr: x and b are already in the solver's layout.
 static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vector<double>& hist,
                        std::vector<int>& cit, double* res0_out, int* ncyc_out, int* nref_out, double* rho_out,
                        double* ux, const double* ub) {
@@ -554,43 +556,55 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   const bool big = f.ny >= 512;   // refinement kernels' tile height RESID_TY_BIG, else RESID_TY
   MarchGrid G = Ops<double>::march_grid(f, big ? RESID_TY_BIG : RESID_TY, false);
   int nb = G.gx * G.gy * G.nch;
-  // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x, x2
   // the caller's padded layout (pitches nx+2, (nx+2)(ny+2); offset of cell (1,1,1))
   Lvl<double> U = f.L();
   U.px = f.nx + 2;
   U.pxy = (long)(f.nx + 2) * (f.ny + 2);
   const long uo = 1 + U.px + U.pxy;
+  // where x and b are read from: the caller's arrays, or the solver's layout
+  const Lvl<double> LB = ux ? U : f.L();
+  const double* bsrc = ux ? ub + uo : f.b;
+  bool x_caller = false;   // x lives in the caller's array only (ux)
   c->sol_out_done = false;
-  // r32 = b - A x and its norm, x and b read from the caller's padded
-  // arrays and stored in the solver's layout (x into x2, then swap)
+  // r32 = b - A x and its norm from the caller's arrays; x stays there
   auto start_from_caller = [&]() {
     if (big)
       hipLaunchKernelGGL((k_refine_start<RESID_TY_BIG>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream,
-                         ux + uo, ub + uo, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
+                         ux + uo, ub + uo, f32.b, f.L(), U, G, c->d_partial);
     else
       hipLaunchKernelGGL((k_refine_start<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, ux + uo,
-                         ub + uo, f.x2, f.b, f32.b, f.L(), U, G, c->d_partial);
+                         ub + uo, f32.b, f.L(), U, G, c->d_partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
-    std::swap(f.x, f.x2);
+    x_caller = true;
+    c->sol_out_done = true;
   };
   // x2 = x + e; r32 = float(b - A x2); partial |b - A x2|^2; then swap x,
-  // x2.  last: x + e goes only into the caller's array, with its norm; if
-  // the step then misses the tolerance, start_from_caller restores x and r32
+  // x2 (x read from the caller's array right after a start).  last: x + e
+  // goes only into the caller's array, with its norm; if the step then
+  // misses the tolerance, start_from_caller forms r32 from there.
   auto refine_update = [&](bool last) {
     if (last && big)
       hipLaunchKernelGGL((k_refine_last<RESID_TY_BIG>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream,
-                         f.x, f32.x, f.b, (float*)nullptr, (double*)nullptr, f.L(), U, G, c->d_partial, ux + uo);
+                         f.x, f32.x, bsrc, f.L(), LB, U, G, c->d_partial, ux + uo);
     else if (last)
       hipLaunchKernelGGL((k_refine_last<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, f.x,
-                         f32.x, f.b, (float*)nullptr, (double*)nullptr, f.L(), U, G, c->d_partial, ux + uo);
-    else if (big)
-      hipLaunchKernelGGL((k_resid_march<double, RESID_TY_BIG, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0,
-                         c->stream, f.x, f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
-    else
-      hipLaunchKernelGGL((k_resid_march<double, RESID_TY, 2>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0,
-                         c->stream, f.x, f32.x, f.b, (double*)nullptr, f32.b, f.x2, f.L(), G, c->d_partial);
+                         f32.x, bsrc, f.L(), LB, U, G, c->d_partial, ux + uo);
+    else {
+      const double* xs = x_caller ? ux + uo : f.x;
+      const Lvl<double> LX = x_caller ? U : f.L();
+      if (big)
+        hipLaunchKernelGGL((k_refine_step<RESID_TY_BIG>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, xs,
+                           f32.x, bsrc, f32.b, f.x2, f.L(), LX, LB, G, c->d_partial);
+      else
+        hipLaunchKernelGGL((k_refine_step<RESID_TY>), dim3((unsigned)nb), dim3(MARCH_THREADS), 0, c->stream, xs,
+                           f32.x, bsrc, f32.b, f.x2, f.L(), LX, LB, G, c->d_partial);
+    }
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, nb, c->d_scalars);
-    if (!last) std::swap(f.x, f.x2);
+    if (!last) {
+      std::swap(f.x, f.x2);
+      x_caller = false;
+    }
+    c->sol_out_done = last;
   };
   // r32 = b - A x and res0, with e = 0
   if (ux) {   // x and b come from the caller's padded arrays and are stored in the solver's layout
@@ -656,8 +670,9 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     }
     // predicted to stop here: the previous per-cycle factor over this step's cycles, factor-2 margin
     const bool stop_likely = fcyc > 0.0 && rho_prev * std::pow(fcyc, nin) < 2.0 * tol;
-    // (never into an array the entry point will reject: its ghost flag has arrived with the first norm)
-    const bool last = stop_likely && ux != nullptr && *c->ghost_flag_host() == 0;
+    // (never into an array the entry point will reject: its ghost flag has
+    // arrived with the first norm; never while x is read from that array)
+    const bool last = stop_likely && ux != nullptr && !x_caller && *c->ghost_flag_host() == 0;
     refine_update(last);
     rc = read_norm(!stop_likely && s < c->opt.max_refine && ncyc < c->ncycle);
     if (rc) return rc;
@@ -673,10 +688,7 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
       *rho_out = rho;
       return diverged(ncyc, rho);
     }
-    if (rho < tol) {
-      c->sol_out_done = last;
-      break;
-    }
+    if (rho < tol) break;
     if (last) start_from_caller();   // mispredicted: x + e is in the caller's array only
     if (rho > 0.0 && rho_prev > 0.0) fcyc = std::pow(rho / rho_prev, 1.0 / nin);
   }
@@ -792,10 +804,12 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
   DevLevel<double>& f = c->h64.lv[0];
   int rc = PEPSMG_OK;
   int flag = 0;
-  // Mixed precision, and FP64 with the fused smoother's norm fold: the first
-  // refinement step or smoothing iteration reads the caller's arrays and
-  // leaves x and b in the solver's layout, so only the ghost layer of sol is
-  // checked here; other FP64 paths: copy-in passes (with the same check).
+  // Mixed precision, and FP64 with the fused smoother's norm fold: the solve
+  // reads the caller's arrays itself (FP64: the first smoothing iteration,
+  // which leaves x and b in the solver's layout; mixed precision: every
+  // refinement step reads b there, and x until the first update), so only
+  // the ghost layer of sol is checked here; other FP64 paths: copy-in passes
+  // (with the same check).
   const bool direct = c->opt.precision == 1 || (c->opt.fused_smoother && c->opt.smoother == 0 && maxiteration > 0 &&
                                                 c->ncycle > 0);
   hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
@@ -826,9 +840,12 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
   if (!rc) {
     if (!(direct && c->sol_out_done)) unpack(c->stream, d_sol, f.x, f, 0);   // else the last step stored it
     if (d_rsd) {
-      // a last refinement step stored x + e into sol only: take x from there
-      // (the FP64 path stored it into both)
-      if (c->opt.precision == 1 && c->sol_out_done) pack(c->stream, f.x, d_sol, f, (int*)nullptr);
+      // mixed precision: b was only read from rhs, and x is in sol when the
+      // solve ended there (the FP64 path stored x and b into both)
+      if (c->opt.precision == 1) {
+        pack(c->stream, f.b, d_rhs, f, (int*)nullptr);
+        if (c->sol_out_done) pack(c->stream, f.x, d_sol, f, (int*)nullptr);
+      }
       Ops<double> op{c->stream};
       op.residual(f, f.r, f.x, f.b);                                   // :1970
       unpack(c->stream, d_rsd, f.r, f, 1);
