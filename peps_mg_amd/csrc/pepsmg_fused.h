@@ -29,7 +29,9 @@
 // 2048 cells or more, XCD-aware (fused_xcd_bid).
 //
 // MODE 0: plain.  MODE 1: x_in = 0 (the zeroed coarse x of
-// src/multigrid.f90:1937; nothing is read).  MODE 2: plain, and each block
+// src/multigrid.f90:1937; x is not read): the LDS planes and loads that
+// stage x_in stage b instead, so b arrives as whole pairs once per plane
+// (the other modes load it per colour into registers).  MODE 2: plain, and each block
 // also writes the partial sum of (b - A x_in)^2 over its tile cells (the
 // convergence norm of src/multigrid.f90:1970-1971 for the x the iteration
 // starts from): the residual of plane m is evaluated in the red phase of
@@ -139,6 +141,7 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   constexpr bool NORM = MODE == 2 || MODE == 3;   // partial sums of (b - A x_in)^2
   constexpr bool CALLER_IN = MODE == 3;           // x_in and b in the caller's layout U
   constexpr bool CALLER_OUT = MODE == 4;          // x_out also into the caller's layout U
+  constexpr bool ZB = MODE == 1;                  // x_in = 0; the staged planes hold b
   using S = FusedShape<T, TX, TY>;
   constexpr int NT = S::NT, RP = S::RP, QY = S::QY, TP = S::TP;
   constexpr int NXL = S::NXL, NRED = S::NRED, NBLK = S::NBLK, PLANE = S::PLANE;
@@ -250,7 +253,7 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   V2 xp[PF][NXL];
   T br[PF][NRED], bk[PF][NBLK];
   auto issue_x = [&](V2* dst, int m) {
-    const T* base = xi + (long)(min(max(m, 1), L.nz) - 1) * ipxy;
+    const T* base = (ZB ? b : xi) + (long)(min(max(m, 1), L.nz) - 1) * ipxy;
 #pragma unroll
     for (int q = 0; q < NXL; q++) {
       if (CALLER_IN) {
@@ -273,12 +276,14 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
     }
   };
   auto issue_bred = [&](T* dst, int m, auto Zc) {
+    if (ZB) return;
     constexpr int Z = decltype(Zc)::value;
     const T* base = b + (long)(min(max(m, 1), L.nz) - 1) * ipxy;
 #pragma unroll
     for (int q = 0; q < NRED; q++) dst[q] = base[r_bo[q][Z]];
   };
   auto issue_bblk = [&](T* dst, int k, auto Zc) {
+    if (ZB) return;
     constexpr int Z = decltype(Zc)::value;
     const T* base = b + (long)(min(max(k, 1), L.nz) - 1) * ipxy;
 #pragma unroll
@@ -303,8 +308,12 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
         const int ss = r_ss[q][Z], oo = r_oo[q][Z];
         T c1 = r_c1[q][Z], c2 = r_c2[q][Z], c3 = r_c3[q], c4 = r_c4[q];
         T c0 = -(r_s4[q][Z] + c5 + c6);
-        T temp = c1 * X0[oo] + c2 * X0[oo + 1] + c3 * X0[ss - RP] + c4 * X0[ss + RP] + c5 * Xm[ss] + c6 * Xp[ss];
-        v = sor_update(omega, bv[q], temp, c0, X0[ss]);
+        if (ZB) {   // black neighbours and the old value are 0 (their products sum to +0)
+          v = sor_update(omega, X0[ss], T(0), c0, T(0));
+        } else {
+          T temp = c1 * X0[oo] + c2 * X0[oo + 1] + c3 * X0[ss - RP] + c4 * X0[ss + RP] + c5 * Xm[ss] + c6 * Xp[ss];
+          v = sor_update(omega, bv[q], temp, c0, X0[ss]);
+        }
       }
       if (t + q * NT < QP) r[r_q[q]] = v;
     }
@@ -329,7 +338,8 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
         T c1 = b_c1[q][Z], c2 = b_c2[q][Z], c3 = b_c3[q], c4 = b_c4[q];
         T c0 = -(b_s4[q][Z] + c5 + c6);
         T temp = c1 * r0[ow] + c2 * r0[ow + 1] + c3 * r0[row - RP] + c4 * r0[row + RP] + c5 * rm[row] + c6 * rp[row];
-        vb = sor_update(omega, bv[q], temp, c0, X0[b_xc[q][Z]]);
+        if (ZB) vb = sor_update(omega, X0[b_xc[q][Z]], temp, c0, T(0));   // X0: b of plane k
+        else vb = sor_update(omega, bv[q], temp, c0, X0[b_xc[q][Z]]);
       }
       T vr = r0[row];
       const bool bf = Z ? !b_odd[q] : b_odd[q];   // black is the first cell iff gj + k odd
@@ -397,22 +407,24 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
 
   using Z0 = std::integral_constant<int, 0>;
   using Z1 = std::integral_constant<int, 1>;
-  // prologue: planes k0-2 .. k0 staged; k0+1 .. k0+PF in flight (k0 is odd)
-  if (MODE == 1) {
-    for (int e = t; e < 8 * PLANE; e += NT) sx[e] = T(0);
-  } else {
+  // prologue: planes k0-2 .. k0 staged; k0+1 .. k0+PF in flight (k0 is
+  // odd).  The three staged planes' loads are all issued before the first is
+  // staged: one memory round trip instead of three (small levels are bound
+  // by that latency).
+  {
+    V2 pa[NXL], pb[NXL];
     issue_x(xp[0], k0 - 2);
+    issue_x(pa, k0 - 1);
+    issue_x(pb, k0);
     stage_x(xp[0], k0 - 2);
-    issue_x(xp[0], k0 - 1);
-    stage_x(xp[0], k0 - 1);
-    issue_x(xp[0], k0);
-    stage_x(xp[0], k0);
-    if (PF == 2) {
-      issue_x(xp[1 % PF], k0 + 1);
-      issue_x(xp[0], k0 + 2);
-    } else {
-      issue_x(xp[0], k0 + 1);
-    }
+    stage_x(pa, k0 - 1);
+    stage_x(pb, k0);
+  }
+  if (PF == 2) {
+    issue_x(xp[1 % PF], k0 + 1);
+    issue_x(xp[0], k0 + 2);
+  } else {
+    issue_x(xp[0], k0 + 1);
   }
   if (PF == 2) {
     issue_bred(br[1 % PF], k0 - 1, Z0());
@@ -439,10 +451,8 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
     lds_barrier();
     if (m - 1 >= k0) black_plane(bk[BB], m - 1, std::integral_constant<int, 1 - ZM>());
     if (m >= k0) issue_bblk(bk[BB], m - 1 + PF, std::integral_constant<int, (1 - ZM + PF) & 1>());
-    if (MODE != 1) {
-      stage_x(xp[B], m + 2);
-      issue_x(xp[B], m + 2 + PF);
-    }
+    stage_x(xp[B], m + 2);
+    issue_x(xp[B], m + 2 + PF);
   };
   for (int m = k0 - 1; m <= k1; m += 2) {
     step(m, std::integral_constant<int, 1>());
