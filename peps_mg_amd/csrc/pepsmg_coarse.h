@@ -68,20 +68,19 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
   // branch around the loads; coarse values outside the domain are zeroed
   // when staged, fine values of cells outside are never stored.
   T cpre[NCL];
-  int cq = 0;
-  auto issue_c = [&](int kc) {
-    cq = kc;
+  auto issue_c = [&](T* dst, int kc) {
     const int kcc = min(max(kc, 1), Cc.nz);
 #pragma unroll
     for (int q = 0; q < NCL; q++) {
       int e = min(t + q * NT, CX * CY - 1);
       int cy = e / CX, cx = e - cy * CX;
       int gi = min(max(ic0 - 1 + cx, 1), Cc.nx), gj = min(max(jc0 - 1 + cy, 1), Cc.ny);
-      cpre[q] = xc[lidx(Cc, gi, gj, kcc)];
+      dst[q] = xc[lidx(Cc, gi, gj, kcc)];
     }
   };
-  auto stage_c = [&](int kc) {
-    const bool vz = cq >= 1 && cq <= Cc.nz;
+  // src: the values issue_c loaded for coarse plane kc
+  auto stage_c = [&](const T* src, int kc) {
+    const bool vz = kc >= 1 && kc <= Cc.nz;
 #pragma unroll
     for (int q = 0; q < NCL; q++) {
       int e = t + q * NT;
@@ -89,7 +88,7 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
         int cy = e / CX, cx = e - cy * CX;
         int gi = ic0 - 1 + cx, gj = jc0 - 1 + cy;
         bool v = vz && gi >= 1 && gi <= Cc.nx && gj >= 1 && gj <= Cc.ny;
-        (&sc[kc & 3][0][0])[e] = v ? cpre[q] : T(0);
+        (&sc[kc & 3][0][0])[e] = v ? src[q] : T(0);
       }
     }
   };
@@ -106,13 +105,16 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
     }
   };
 
-  issue_c(kc0 - 1);
-  stage_c(kc0 - 1);
-  issue_c(kc0);
-  stage_c(kc0);
-  issue_c(kc0 + 1);
-  stage_c(kc0 + 1);
-  issue_c(kc0 + 2);
+  {   // the three staged planes' loads in flight before the first is staged
+    T ca[NCL], cb[NCL];
+    issue_c(cpre, kc0 - 1);
+    issue_c(ca, kc0);
+    issue_c(cb, kc0 + 1);
+    stage_c(cpre, kc0 - 1);
+    stage_c(ca, kc0);
+    stage_c(cb, kc0 + 1);
+  }
+  issue_c(cpre, kc0 + 2);
   issue_f(xpre[0], 2 * kc0 - 1);
   issue_f(xpre[1], 2 * kc0);
   // coarse-plane step, run twice per iteration so the compiler's wait
@@ -157,8 +159,8 @@ __device__ __forceinline__ void prolong_march_block(int bid, T* __restrict__ xf,
         else xf[q0] = xv[q].x;
       }
     }
-    stage_c(kc + 2);
-    issue_c(kc + 3);
+    stage_c(cpre, kc + 2);
+    issue_c(cpre, kc + 3);
   };
   int kc = kc0;
   for (; kc + 1 < kc1; kc += 2) {

@@ -205,10 +205,17 @@ __device__ __forceinline__ void resid_march_block(int bid, const T* __restrict__
     st.cgp = b;
     st.beta = beta;
   }
-  st.issue(x, LXI, e, L, i0, j0, k0 - 1);
-  st.stage(sx[(k0 - 1) & 3], L, i0, j0);
-  st.issue(x, LXI, e, L, i0, j0, k0);
-  st.stage(sx[k0 & 3], L, i0, j0);
+  {   // both staged planes' loads in flight before the first is staged
+    St s2;
+    if (CGP) {
+      s2.cgp = b;
+      s2.beta = beta;
+    }
+    st.issue(x, LXI, e, L, i0, j0, k0 - 1);
+    s2.issue(x, LXI, e, L, i0, j0, k0);
+    st.stage(sx[(k0 - 1) & 3], L, i0, j0);
+    s2.stage(sx[k0 & 3], L, i0, j0);
+  }
   st.issue(x, LXI, e, L, i0, j0, k0 + 1);
   issue_b(k0);
   double acc = 0.0;
@@ -404,10 +411,13 @@ __device__ __forceinline__ void restrict_march_block(int bid, T* __restrict__ bc
     bpre[1] = ld_stream_v2(b + lidx(L, gi, g1, mc));
   };
 
-  st.issue(x, L, nullptr, L, i0, j0, k0 - 1);
-  st.stage(sx[(k0 - 1) & 3], L, i0, j0);
-  st.issue(x, L, nullptr, L, i0, j0, k0);
-  st.stage(sx[k0 & 3], L, i0, j0);
+  {   // both staged planes' loads in flight before the first is staged
+    St s2;
+    st.issue(x, L, nullptr, L, i0, j0, k0 - 1);
+    s2.issue(x, L, nullptr, L, i0, j0, k0);
+    st.stage(sx[(k0 - 1) & 3], L, i0, j0);
+    s2.stage(sx[k0 & 3], L, i0, j0);
+  }
   st.issue(x, L, nullptr, L, i0, j0, k0 + 1);
   issue_b(k0);
   // residuals of the odd plane kz = 2kc-1: (x child, y child) = (z,z), (p,z), (z,p), (p,p)
