@@ -315,7 +315,7 @@ static void rec_smooth_group(pepsmg_batch* B, Sched& S, std::vector<Hier<T>*>& H
     e.xi = d.x;
     e.b = d.b;
     e.L = d.L();
-    e.G = Ops<T>::fused_grid(d, TX, TY);
+    e.G = Ops<T>::fused_grid(d, TX, TY, 0, true);
     nb += e.G.gx * e.G.gy * e.G.nch;
     tab.push_back(e);
     std::swap(d.x, d.x2);

@@ -71,7 +71,10 @@ struct Ops {
   // of 2 below: levels of 128 planes and fewer are bound by the latency of
   // the march, not by bytes (FP32 smoother at 32^3 / 64^3: 6.8 / 7.5 us
   // against 8.2 / 8.7 us with chunks of 4 planes).
-  static FusedGrid fused_grid(const DevLevel<T>& d, int TX, int TY, int resident_blocks = 0) {
+  // batched: the launch covers many problems at once, so it is bound by
+  // bytes rather than latency and keeps chunks of at least 4 planes (12
+  // from 96 planes on), which re-read fewer halo planes.
+  static FusedGrid fused_grid(const DevLevel<T>& d, int TX, int TY, int resident_blocks = 0, bool batched = false) {
     FusedGrid G;
     G.gx = (d.nx + TX - 1) / TX;
     G.gy = (d.ny + TY - 1) / TY;
@@ -80,10 +83,11 @@ struct Ops {
     if (resident_blocks > 0 && resident_blocks % nxy == 0 && d.nz / (resident_blocks / nxy) >= 4) {
       nch = resident_blocks / nxy;
     } else {
-      nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + 1) / 2));
+      const int min_planes = batched ? 4 : 2, long_from = batched ? 96 : 192;
+      nch = std::max(1L, std::min<long>((2048 + nxy - 1) / nxy, (d.nz + min_planes - 1) / min_planes));
       if (nch < 8 && d.nz >= 64) nch = 8;   // one z-slab per XCD: tile rings hit the XCD's own L2
       if (nch >= 8) nch = (nch + 7) / 8 * 8;
-      if (d.nz >= 192) nch = std::min<long>(nch, std::max<long>(8, d.nz / 12 / 8 * 8));
+      if (d.nz >= long_from) nch = std::min<long>(nch, std::max<long>(8, d.nz / 12 / 8 * 8));
     }
     int kc = (int)((d.nz + nch - 1) / nch);
     if (kc & 1) kc++;                      // even chunks: every chunk starts on an odd plane
