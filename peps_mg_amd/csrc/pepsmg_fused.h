@@ -113,6 +113,14 @@ __device__ __forceinline__ float sor_update(float omega, float bv, float temp, f
   return omega * (bv - temp) * __builtin_amdgcn_rcpf(c0) + (1.0f - omega) * xold;
 }
 
+// Modes 0 and 4 on tiles of 2048 cells or more (one or two blocks per CU
+// either way) load b by whole pairs along with the red values and pass the
+// black cells' b to the black phase through LDS, instead of a load per
+// colour (FP64 512^3 plane iterations 1.4 % faster, mode 4 6 %); the norm
+// modes 2 and 3 ran 18 % slower that way and keep the loads per colour, and
+// mode 1 stages b in the x planes instead.
+constexpr bool rb_pair_b(int mode, int tx, int ty) { return (mode == 0 || mode == 4) && tx * ty >= 2048; }
+
 template <typename T, int TX, int TY>
 struct FusedShape {
   static constexpr int NT = fused_threads(TX, TY);
@@ -137,11 +145,12 @@ template <typename T, int TX, int TY, int MODE, int PF>
 __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __restrict__ xi,
                                                 const T* __restrict__ b, const Lvl<T>& L, T omega, int i0, int j0,
                                                 int k0, int k1, T* __restrict__ sx, T* __restrict__ sr,
-                                                const Lvl<T>& U, T* __restrict__ ext) {
+                                                T* __restrict__ sbk, const Lvl<T>& U, T* __restrict__ ext) {
   constexpr bool NORM = MODE == 2 || MODE == 3;   // partial sums of (b - A x_in)^2
   constexpr bool CALLER_IN = MODE == 3;           // x_in and b in the caller's layout U
   constexpr bool CALLER_OUT = MODE == 4;          // x_out also into the caller's layout U
   constexpr bool ZB = MODE == 1;                  // x_in = 0; the staged planes hold b
+  constexpr bool PB = rb_pair_b(MODE, TX, TY);     // b loaded by whole pairs with the red values
   using S = FusedShape<T, TX, TY>;
   constexpr int NT = S::NT, RP = S::RP, QY = S::QY, TP = S::TP;
   constexpr int NXL = S::NXL, NRED = S::NRED, NBLK = S::NBLK, PLANE = S::PLANE;
@@ -158,12 +167,21 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   bool r_in[NRED][2];
   T r_c1[NRED][2], r_c2[NRED][2], r_s4[NRED][2], r_c3[NRED], r_c4[NRED];
   int r_ss[NRED][2], r_oo[NRED][2], r_bo[NRED][2], r_q[NRED];
+  // PB: offset of the pair's b (in bounds), the tile slot of its black cell
+  // (-1 on the ring) and whether the red cell is the pair's first
+  int r_po[PB ? NRED : 1], r_bt[PB ? NRED : 1];
+  bool r_first[PB ? NRED : 1][2];
 #pragma unroll
   for (int q = 0; q < NRED; q++) {
     int e = min(t + q * NT, QP - 1);
     int qy = e / RP, p = e - qy * RP;
     int gj = j0 - 1 + qy, ia = i0 - 2 + 2 * p;
     bool row = t + q * NT < QP && gj >= 1 && gj <= L.ny;
+    if (PB) {
+      // an odd column <= nx keeps the 2-element load inside the padded row
+      r_po[q] = (row && ia >= 1 && ia <= L.nx) ? (gj - 1) * ipx + (ia - 1) : 0;
+      r_bt[q] = (t + q * NT < QP && qy >= 1 && qy <= TY && p >= 1 && p <= TP) ? (qy - 1) * TP + (p - 1) : -1;
+    }
     // the first cell is used only from column i0-1 on, the second up to i0+TX
     bool ina = row && p > 0 && ia >= 1 && ia <= L.nx;
     bool inb = row && p < RP - 1 && ia + 1 >= 1 && ia + 1 <= L.nx;
@@ -175,6 +193,7 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
 #pragma unroll
     for (int z = 0; z < 2; z++) {
       bool first = ((gj + z) & 1) == 0;     // red is the first cell iff gj + k even
+      if (PB) r_first[q][z] = first;
       bool in = first ? ina : inb;
       int ic = first ? ia : ia + 1;
       T c1 = in ? L.cwx[ic] : T(0), c2 = in ? L.cex[ic] : T(0);
@@ -251,7 +270,8 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   // read.  With no branch or select next to a load the compiler waits only
   // for the loads a plane needs instead of draining the prefetches.
   V2 xp[PF][NXL];
-  T br[PF][NRED], bk[PF][NBLK];
+  T br[PB ? 1 : PF][NRED], bk[PB ? 1 : PF][NBLK];
+  V2 brp[PB ? PF : 1][NRED];
   auto issue_x = [&](V2* dst, int m) {
     const T* base = (ZB ? b : xi) + (long)(min(max(m, 1), L.nz) - 1) * ipxy;
 #pragma unroll
@@ -275,15 +295,24 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
       }
     }
   };
-  auto issue_bred = [&](T* dst, int m, auto Zc) {
+  // b of the red cells of plane m into buffer slot (PB: whole pairs)
+  auto issue_bred = [&](auto slot_c, int m, auto Zc) {
     if (ZB) return;
+    constexpr int slot = decltype(slot_c)::value;
     constexpr int Z = decltype(Zc)::value;
     const T* base = b + (long)(min(max(m, 1), L.nz) - 1) * ipxy;
 #pragma unroll
-    for (int q = 0; q < NRED; q++) dst[q] = base[r_bo[q][Z]];
+    for (int q = 0; q < NRED; q++) {
+      if (PB) {
+        if (CALLER_IN) brp[slot][q] = ld_pair_unal(base + r_po[q]);
+        else brp[slot][q] = *reinterpret_cast<const V2*>(base + r_po[q]);
+      } else {
+        br[slot][q] = base[r_bo[q][Z]];
+      }
+    }
   };
   auto issue_bblk = [&](T* dst, int k, auto Zc) {
-    if (ZB) return;
+    if (ZB || PB) return;
     constexpr int Z = decltype(Zc)::value;
     const T* base = b + (long)(min(max(k, 1), L.nz) - 1) * ipxy;
 #pragma unroll
@@ -291,8 +320,12 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
   };
 
   // new red values of plane m (parity Z) on the tile and its ring
-  auto red_plane = [&](const T* bv, int m, auto Zc) {
+  // PB: the black cells' b of the tile pairs go to sbk (plane parity slot)
+  // for black_plane one step later
+  auto red_plane = [&](auto slot_c, int m, auto Zc) {
+    constexpr int slot = decltype(slot_c)::value;
     constexpr int Z = decltype(Zc)::value;
+    const T* bv = br[PB ? 0 : slot];
     const T* X0 = sx + (m & 3) * PLANE;
     const T* Xm = sx + ((m - 1) & 3) * PLANE;
     const T* Xp = sx + ((m + 1) & 3) * PLANE;
@@ -312,16 +345,26 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
           v = sor_update(omega, X0[ss], T(0), c0, T(0));
         } else {
           T temp = c1 * X0[oo] + c2 * X0[oo + 1] + c3 * X0[ss - RP] + c4 * X0[ss + RP] + c5 * Xm[ss] + c6 * Xp[ss];
-          v = sor_update(omega, bv[q], temp, c0, X0[ss]);
+          T bred = bv[q];
+          if (PB) {
+            const T p0 = brp[slot][q].x, p1 = brp[slot][q].y;
+            bred = r_first[q][Z] ? p0 : p1;
+          }
+          v = sor_update(omega, bred, temp, c0, X0[ss]);
         }
       }
       if (t + q * NT < QP) r[r_q[q]] = v;
+      if (PB && r_bt[q] >= 0) {
+        const T p0 = brp[slot][q].x, p1 = brp[slot][q].y;
+        sbk[(m & 1) * TY * TP + r_bt[q]] = r_first[q][Z] ? p1 : p0;
+      }
     }
   };
 
   // black values of plane k (parity Z) on the tile, stored with the red ones
   auto black_plane = [&](const T* bv, int k, auto Zc) {
     constexpr int Z = decltype(Zc)::value;
+    const T* sb = sbk + (k & 1) * TY * TP;   // PB: b of the tile's black cells of plane k
     const T* r0 = sr + (k & 3) * QP;
     const T* rm = sr + ((k - 1) & 3) * QP;
     const T* rp = sr + ((k + 1) & 3) * QP;
@@ -339,7 +382,7 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
         T c0 = -(b_s4[q][Z] + c5 + c6);
         T temp = c1 * r0[ow] + c2 * r0[ow + 1] + c3 * r0[row - RP] + c4 * r0[row + RP] + c5 * rm[row] + c6 * rp[row];
         if (ZB) vb = sor_update(omega, X0[b_xc[q][Z]], temp, c0, T(0));   // X0: b of plane k
-        else vb = sor_update(omega, bv[q], temp, c0, X0[b_xc[q][Z]]);
+        else vb = sor_update(omega, PB ? sb[min(t + q * NT, TY * TP - 1)] : bv[q], temp, c0, X0[b_xc[q][Z]]);
       }
       T vr = r0[row];
       const bool bf = Z ? !b_odd[q] : b_odd[q];   // black is the first cell iff gj + k odd
@@ -427,11 +470,11 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
     issue_x(xp[0], k0 + 1);
   }
   if (PF == 2) {
-    issue_bred(br[1 % PF], k0 - 1, Z0());
-    issue_bred(br[0], k0, Z1());
+    issue_bred(std::integral_constant<int, 1 % PF>(), k0 - 1, Z0());
+    issue_bred(std::integral_constant<int, 0>(), k0, Z1());
     issue_bblk(bk[0], k0, Z1());
   } else {
-    issue_bred(br[0], k0 - 1, Z0());
+    issue_bred(std::integral_constant<int, 0>(), k0 - 1, Z0());
   }
 
   // One plane step for plane m: red(m) | barrier | black(m-1), stage x(m+2).
@@ -442,14 +485,14 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
     constexpr int ZM = 1 - P;                       // parity of m
     constexpr int B = PF == 2 ? P : 0, BB = PF == 2 ? 1 - P : 0;
     lds_barrier();
-    red_plane(br[B], m, std::integral_constant<int, ZM>());     // needs b(m), x planes m-1..m+1
-    issue_bred(br[B], m + PF, std::integral_constant<int, (ZM + PF) & 1>());
+    red_plane(std::integral_constant<int, B>(), m, std::integral_constant<int, ZM>());   // b(m), x planes m-1..m+1
+    issue_bred(std::integral_constant<int, B>(), m + PF, std::integral_constant<int, (ZM + PF) & 1>());
     if (NORM) {
       if (m >= k0 && m < k1) norm_plane(m, std::integral_constant<int, ZM>());
       issue_bn(m + 1);
     }
     lds_barrier();
-    if (m - 1 >= k0) black_plane(bk[BB], m - 1, std::integral_constant<int, 1 - ZM>());
+    if (m - 1 >= k0) black_plane(bk[PB ? 0 : BB], m - 1, std::integral_constant<int, 1 - ZM>());
     if (m >= k0) issue_bblk(bk[BB], m - 1 + PF, std::integral_constant<int, (1 - ZM + PF) & 1>());
     stage_x(xp[B], m + 2);
     issue_x(xp[B], m + 2 + PF);
@@ -472,6 +515,7 @@ __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, cons
   // then second cells (i+1); new red values (4 slots), one per pair
   __shared__ T sx[8 * S::PLANE];
   __shared__ T sr[4 * S::QY * S::RP];
+  __shared__ T sbk[rb_pair_b(MODE, TX, TY) ? 2 * TY * S::TP : 1];   // two planes of black-cell b
 
   const int bz = bid % G.nch;
   const int rxy = bid / G.nch;
@@ -479,7 +523,7 @@ __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, cons
   const int j0 = 1 + (rxy / G.gx) * TY;
   const int k0 = 1 + bz * G.kchunk;              // odd: kchunk is even
   const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
-  const double acc = rb_fused_body<T, TX, TY, MODE, PF>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr, U, ext);
+  const double acc = rb_fused_body<T, TX, TY, MODE, PF>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr, sbk, U, ext);
   if (MODE == 2 || MODE == 3) {
     __shared__ double sh[fused_threads(TX, TY) / 64];
     const double s = block_sum(acc, sh);
