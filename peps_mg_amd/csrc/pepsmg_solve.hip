@@ -607,7 +607,7 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
     c->sol_out_done = last;
   };
   // r32 = b - A x and res0, with e = 0
-  if (ux) {   // x and b come from the caller's padded arrays and are stored in the solver's layout
+  if (ux) {   // x and b are read from the caller's padded arrays
     start_from_caller();
   } else {
     hipMemsetAsync(f32.x, 0, f32.ncell * sizeof(float), c->stream);
@@ -876,7 +876,7 @@ int pepsmg_multigrid_solve_vcycle(pepsmg_ctx* c, double* sol, double* rsd, const
   }
   HIP_TRY(hipMemcpy(c->d_pad, sol, np * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_pad2, rhs, np * sizeof(double), hipMemcpyHostToDevice));
-  // the rhs staging receives rsd after the solve (b is in the solver's layout by then)
+  // the rhs staging receives rsd after the solve (b is copied into the solver's layout before rsd is formed)
   rc = pepsmg_multigrid_solve_vcycle_device(c, c->d_pad, rsd ? c->d_pad2 : nullptr, c->d_pad2, maxiteration,
                                             tolerance, omega_sor, nullptr, st);
   if (rc) return rc;
