@@ -192,6 +192,7 @@ k_prolong_march(T* __restrict__ xf, const T* __restrict__ xc, Lvl<T> F, Lvl<T> C
 // into LDS at the start, so no phase waits on a global load.
 constexpr int PC_MAXLEV = 8;
 constexpr int PC_THREADS = 1024;
+constexpr int PC_CINV_MAX = 128;   // largest coarsest level solved by its inverse (FP32 inner cycles)
 
 template <typename T>
 struct PCArgs {
@@ -213,6 +214,11 @@ struct PCArgs {
   T* gxs;                         // global x_s (output)
   const T* gbs;                   // global b_s (input)
   int* iters_slot;
+  // FP32 inner cycles of the mixed-precision solve: the coarsest level's
+  // inverse operator, transposed (n x n doubles, entry k n + l = (A^-1)_lk),
+  // for an exact coarsest solve in place of the red/black iterations; null
+  // otherwise (the FP64 solve keeps the reference's iterations)
+  const double* cinv;
 };
 
 
@@ -583,7 +589,16 @@ __device__ __forceinline__ void coarse_vcycle_block(const PCArgs<T>& A) {
     const double* b = smd + A.bcoff;
     int n = L.nx * L.ny * L.nz;
     pc_zero(x, n);
-    if (n == 1) {
+    if (A.cinv && n <= PC_CINV_MAX) {   // x = A^-1 b, one lane per cell
+      if ((int)threadIdx.x < n) {
+        double v = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < n; k++) v += A.cinv[(long)k * n + threadIdx.x] * b[k];
+        x[threadIdx.x] = v;
+      }
+      if (threadIdx.x == 0 && A.iters_slot) *A.iters_slot = 0;
+      __syncthreads();
+    } else if (n == 1) {
       if (threadIdx.x == 0) {
         x[0] = b[0] / diag(L.cwx[1], L.cex[1], L.cwy[1], L.cey[1], L.cwz[1], L.cez[1]);
         if (A.iters_slot) *A.iters_slot = 0;

@@ -174,6 +174,7 @@ struct pepsmg_ctx {
   hipEvent_t ev_entry = nullptr;   // orders the own stream after the caller's default-stream work
   bool sol_out_done = false;       // the caller's sol holds the solution (stored there by the last step, or never left it)
   pepsmg::KernelTimer timer;       // measurement hook (pepsmg_kernel_timing)
+  double* d_cinv = nullptr;        // coarsest level's inverse operator (mixed precision), built on first use
   // host copy of the ghost-cell flag of the solve in progress (read back with its first norm)
   int* ghost_flag_host() { return reinterpret_cast<int*>(h_scalars + 7); }
 

@@ -225,7 +225,8 @@ void pepsmg_ctx::release() {
   hipFree(d_citers);
   hipFree(d_pad);
   hipFree(d_pad2);
-  d_partial = d_scalars = d_pad = d_pad2 = nullptr;
+  hipFree(d_cinv);
+  d_partial = d_scalars = d_pad = d_pad2 = d_cinv = nullptr;
   d_flag = d_citers = nullptr;
   if (ev0) hipEventDestroy(ev0);
   if (ev1) hipEventDestroy(ev1);
@@ -320,6 +321,73 @@ static void coarse_solve_level(pepsmg_ctx* c, Hier<float>& h, float omega, doubl
   hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, cf.x, cd.x, n);
 }
 
+// The inverse of the coarsest level's operator (c0 on the diagonal, the
+// neighbour coefficients beside it; ghost neighbours are 0), by Gauss-Jordan
+// elimination with partial pivoting in FP64, stored transposed on the device
+// (c->d_cinv).  The FP32 inner V-cycles of the mixed-precision solve apply it
+// as their coarsest solve; levels larger than PC_CINV_MAX cells keep the
+// red/black iterations.
+static int ensure_coarse_inverse(pepsmg_ctx* c) {
+  if (c->d_cinv) return PEPSMG_OK;
+  const HostLevel& h = c->host[c->nlevel];
+  const int nx = h.n[0], ny = h.n[1], nz = h.n[2], n = nx * ny * nz;
+  if (n > PC_CINV_MAX) return PEPSMG_OK;
+  std::vector<double> a((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        const int q = (i - 1) + nx * ((j - 1) + ny * (k - 1));
+        const double cw = h.cw[0][i], ce = h.ce[0][i], cs = h.cw[1][j], cn = h.ce[1][j], cb = h.cw[2][k],
+                     ct = h.ce[2][k];
+        double* row = &a[(size_t)q * n];
+        row[q] = -(((((cw + ce) + cs) + cn) + cb) + ct);
+        if (i > 1) row[q - 1] = cw;
+        if (i < nx) row[q + 1] = ce;
+        if (j > 1) row[q - nx] = cs;
+        if (j < ny) row[q + nx] = cn;
+        if (k > 1) row[q - nx * ny] = cb;
+        if (k < nz) row[q + nx * ny] = ct;
+      }
+  for (int q = 0; q < n; q++) inv[(size_t)q * n + q] = 1.0;
+  for (int col = 0; col < n; col++) {
+    int piv = col;
+    for (int r = col + 1; r < n; r++)
+      if (std::fabs(a[(size_t)r * n + col]) > std::fabs(a[(size_t)piv * n + col])) piv = r;
+    if (!(std::fabs(a[(size_t)piv * n + col]) > 0.0)) return PEPSMG_OK;   // singular: keep the iterations
+    if (piv != col)
+      for (int k = 0; k < n; k++) {
+        std::swap(a[(size_t)piv * n + k], a[(size_t)col * n + k]);
+        std::swap(inv[(size_t)piv * n + k], inv[(size_t)col * n + k]);
+      }
+    const double d = a[(size_t)col * n + col];
+    for (int k = 0; k < n; k++) {
+      a[(size_t)col * n + k] /= d;
+      inv[(size_t)col * n + k] /= d;
+    }
+    for (int r = 0; r < n; r++) {
+      if (r == col) continue;
+      const double f = a[(size_t)r * n + col];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; k++) {
+        a[(size_t)r * n + k] -= f * a[(size_t)col * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)col * n + k];
+      }
+    }
+  }
+  std::vector<double> t((size_t)n * n);
+  for (int l = 0; l < n; l++)
+    for (int k = 0; k < n; k++) t[(size_t)k * n + l] = inv[(size_t)l * n + k];
+  double* d = nullptr;
+  HIP_TRY(hipMalloc(&d, t.size() * sizeof(double)));
+  hipError_t e = hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(d);
+    return hip_fail(e, "hipMemcpy (coarsest inverse)");
+  }
+  c->d_cinv = d;
+  return PEPSMG_OK;
+}
+
 // The down/up legs of one V-cycle below the fine level, for hierarchy h
 // whose level-1 right-hand side has been restricted already
 // (src/multigrid.f90:1936-1962).  Levels from the switch level s down run
@@ -342,6 +410,7 @@ static void vcycle_coarse_part(pepsmg_ctx* c, Ops<T>& op, Hier<T>& h, CgBufs<T>*
       }
     }
   }
+  if constexpr (std::is_same<T, float>::value) A.cinv = c->d_cinv;   // exact coarsest solve (inner cycles)
   int last = std::min(s, L) - 1;   // last level of the per-level down-leg
   for (int l = 1; l <= last; l++) {
     DevLevel<T>& d = h.lv[l];
@@ -549,6 +618,10 @@ static int solve_mixed(pepsmg_ctx* c, int nu, double tol, double omega, std::vec
   if (c->opt.smoother == 1) {
     cg = &c->cg32;
     int rc = cg->ensure(g);
+    if (rc) return rc;
+  }
+  {
+    int rc = ensure_coarse_inverse(c);
     if (rc) return rc;
   }
   DevLevel<double>& f = h.lv[0];
