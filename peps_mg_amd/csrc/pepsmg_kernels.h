@@ -444,20 +444,30 @@ __global__ void k_pack(T* __restrict__ dst, const double* __restrict__ src, Lvl<
 }
 
 // flag = 1 if a padded (0:nx+1, 0:ny+1, 0:nz+1) array has a non-zero ghost
-// cell; one wavefront per (j, k) row: whole rows on the y and z faces, the
-// two end cells of the others.
+// cell.  Work items, visited by a grid-stride loop: the two z faces and the
+// y-face rows of every interior plane, one cell each (consecutive items
+// are consecutive addresses), then the two end cells of every interior row,
+// one row each.
 static __global__ void __launch_bounds__(256)
 k_ghost_check(const double* __restrict__ a, int nx, int ny, int nz, int* __restrict__ flag) {
-  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= (long)(ny + 2) * (nz + 2)) return;
-  const int j = (int)(row % (ny + 2)), k = (int)(row / (ny + 2));
-  const double* r = a + row * (nx + 2);
+  const long px = nx + 2, pxy = px * (ny + 2);
+  const long nface = 2 * pxy + 2L * nz * px;   // z faces, then rows j = 0 and ny+1 of planes 1..nz
+  const long nrow = (long)ny * nz;              // interior rows
   bool bad = false;
-  if (j == 0 || k == 0 || j == ny + 1 || k == nz + 1) {
-    for (int i = lane; i <= nx + 1; i += 64) bad |= r[i] != 0.0;
-  } else if (lane < 2) {
-    bad = r[lane == 0 ? 0 : nx + 1] != 0.0;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nface + nrow; q += (long)gridDim.x * blockDim.x) {
+    if (q < pxy) {
+      bad |= a[q] != 0.0;                                       // plane k = 0
+    } else if (q < 2 * pxy) {
+      bad |= a[(long)(nz + 1) * pxy + (q - pxy)] != 0.0;        // plane k = nz + 1
+    } else if (q < nface) {
+      const long e = q - 2 * pxy, r = e / px, i = e - r * px;   // r: (plane, face row) pairs
+      const long k = 1 + r / 2, j = (r & 1) ? ny + 1 : 0;
+      bad |= a[k * pxy + j * px + i] != 0.0;
+    } else {
+      const long e = q - nface, k = 1 + e / ny, j = 1 + e % ny;
+      const double* row = a + k * pxy + j * px;
+      bad |= (row[0] != 0.0) | (row[nx + 1] != 0.0);
+    }
   }
   if (bad) atomicOr(flag, 1);
 }

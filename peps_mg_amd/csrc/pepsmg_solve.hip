@@ -887,9 +887,9 @@ int pepsmg_multigrid_solve_vcycle_device(pepsmg_ctx* c, double* d_sol, double* d
                                                 c->ncycle > 0);
   hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream);
   if (direct) {
-    const long rows = (long)(f.ny + 2) * (f.nz + 2);
-    hipLaunchKernelGGL(k_ghost_check, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, d_sol, f.nx, f.ny,
-                       f.nz, c->d_flag);
+    const long items = 2L * (f.nx + 2) * (f.ny + 2) + 2L * f.nz * (f.nx + 2) + (long)f.ny * f.nz;
+    const unsigned nb = (unsigned)std::min<long>((items + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_ghost_check, dim3(nb), dim3(256), 0, c->stream, d_sol, f.nx, f.ny, f.nz, c->d_flag);
   } else {
     pack(c->stream, f.x, d_sol, f, c->d_flag);
     pack(c->stream, f.b, d_rhs, f, (int*)nullptr);
