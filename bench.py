@@ -180,6 +180,14 @@ def profiler_child(a, args, tag, timeout=150):
     return d, out
 
 
+def is_smoother(kernel, name):
+    """Whether a traced kernel name is the plain smoother `kernel` or its
+    downward-marching twin (the solver alternates the two)."""
+    name = name.replace(" ", "")
+    k = kernel.replace(" ", "")
+    return k in name or k.replace("k_rb_fused<", "k_rb_fused_down<") in name
+
+
 def measure_traffic(a, kernel):
     """HBM bytes per launch of `kernel` from two rocprofv3 counter passes
     (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch) over this script's
@@ -198,7 +206,7 @@ def measure_traffic(a, kernel):
         vals, copy = [], []
         for r in csv.DictReader(open(path)):
             name = r["Kernel_Name"]
-            if kernel.replace(" ", "") in name.replace(" ", ""):
+            if is_smoother(kernel, name):
                 vals.append((int(r.get("Dispatch_Id", len(vals))), float(r["Counter_Value"])))
             elif "copyBuffer" in name and float(r["Counter_Value"]) > 1e5:
                 copy.append(float(r["Counter_Value"]))
@@ -240,7 +248,7 @@ def trace_smoothers(a, kernels):
     res = {}
     for k in kernels:
         ds = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows
-                    if k.replace(" ", "") in r["Kernel_Name"].replace(" ", ""))
+                    if is_smoother(k, r["Kernel_Name"]))
         if len(ds) < a.smoother_reps:
             return None, f"kernel trace: {len(ds)} dispatches of {k}"
         timed = ds[-a.smoother_reps:]
@@ -823,14 +831,17 @@ def main():
             "traffic": round(traffic) if traffic else None, "traffic_note": traffic_note,
             "traffic_read": round(tr["FETCH_SIZE"]) if tr else None,
             "traffic_write": round(tr["WRITE_SIZE"]) if tr else None,
-            "kernel": kname64, "avg_ms": avg64, "launches": n64, "bytes_per_launch": nbytes64,
-            "timing": f"HIP events on the solve stream around every launch of the kernel on the {n}^3 level "
-                      f"inside {ev_steps} FP64 solves run after the timed ones",
+            "kernel": kname64, "kernel_down": kname64.replace("k_rb_fused<", "k_rb_fused_down<"),
+            "avg_ms": avg64, "launches": n64, "bytes_per_launch": nbytes64,
+            "timing": f"HIP events on the solve stream around every plain iteration on the {n}^3 level inside "
+                      f"{ev_steps} FP64 solves run after the timed ones (both kernels: a plain iteration that "
+                      "follows an upward launch on the level marches down)",
             "stream_ceiling_gbs": round(stream_gbs, 1), "frac_of_stream_ceiling": round(achieved / stream_gbs, 4),
             "back_to_back": {"avg_ms": bb64, "frac": frac(nbytes64, bb64),
                              "note": f"{a.smoother_reps} launches in a row after {a.smoother_reps} untimed ones "
-                                     "(pepsmg_bench_smoother), events around the whole run"}}
+                                     "(pepsmg_bench_smoother, alternating upward and downward), events around the whole run"}}
     roof32 = {"achieved": round(achieved32, 1), "frac": round(achieved32 / HBM_PEAK_GBS, 4), "kernel": kname32,
+              "kernel_down": kname32.replace("k_rb_fused<", "k_rb_fused_down<"),
               "avg_ms": avg32, "launches": n32, "bytes_per_launch": nbytes32,
               "timing": f"HIP events around every launch on the {n}^3 level inside {ev_steps} mixed solves "
                         "(default options) run after the timed ones",

@@ -183,7 +183,9 @@ int pepsmg_vv_dot_device_noalloc(int32_t nx, int32_t ny, int32_t nz, const doubl
  * HIP events on that stream.  *avg_ms receives the mean duration of one
  * launch, *bytes the algorithmic HBM bytes of one launch (each input read
  * once, each output written once), *kernel_name the kernel's symbol (a
- * per-thread buffer, valid until the calling thread's next call). */
+ * per-thread buffer, valid until the calling thread's next call).  As in a
+ * solve, a launch that follows an upward one marches down (the symbol's
+ * k_rb_fused_down twin), so the launches alternate between the two. */
 int pepsmg_bench_smoother(pepsmg_ctx *ctx, int32_t precision, int32_t reps, double omega,
                           double *avg_ms, double *bytes, const char **kernel_name);
 
@@ -191,7 +193,7 @@ int pepsmg_bench_smoother(pepsmg_ctx *ctx, int32_t precision, int32_t reps, doub
  * following solve on ctx bracket the plain red/black smoother launches on its
  * finest level (FP64; in precision 1 the FP32 smoother) with HIP events on
  * the solve stream, and clears the totals; 0 stops.  Up to 256 launches per
- * solve are timed. */
+ * solve are timed, both upward and downward ones (k_rb_fused_down). */
 int pepsmg_kernel_timing(pepsmg_ctx *ctx, int32_t enable);
 
 /* Totals since the last pepsmg_kernel_timing(ctx, 1) (kept when timing is

@@ -141,7 +141,9 @@ struct FusedShape {
 // left-to-right sum), its LDS offsets and the offset of its b value.  The
 // chunk start k0 is odd (even chunk length), so each plane step of the
 // two-step unrolled loop knows its plane parity at compile time.
-template <typename T, int TX, int TY, int MODE, int PF>
+// DOWN (mode 0, PF 1): the march runs from the top of the chunk down; every
+// cell gets the same value (the colours' order fixes it, not the planes').
+template <typename T, int TX, int TY, int MODE, int PF, bool DOWN = false>
 __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __restrict__ xi,
                                                 const T* __restrict__ b, const Lvl<T>& L, T omega, int i0, int j0,
                                                 int k0, int k1, T* __restrict__ sx, T* __restrict__ sr,
@@ -450,6 +452,42 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
 
   using Z0 = std::integral_constant<int, 0>;
   using Z1 = std::integral_constant<int, 1>;
+  if constexpr (DOWN) {
+    static_assert(MODE == 0 && PF == 1, "downward march: plain iterations, one plane of look-ahead");
+    // Mirror of the upward march below: plane step m computes red(m) and
+    // then black(m+1), stages x(m-2) and issues x(m-3).  It starts at the
+    // odd plane mtop >= k1 (k1 is even only at the top of the domain, where
+    // plane nz+2 gives zeros), so every step knows its plane parity at
+    // compile time, and ends at k0-1.
+    const int mtop = k1 | 1;
+    {
+      V2 pa[NXL], pb[NXL];
+      issue_x(xp[0], mtop + 1);
+      issue_x(pa, mtop);
+      issue_x(pb, mtop - 1);
+      stage_x(xp[0], mtop + 1);
+      stage_x(pa, mtop);
+      stage_x(pb, mtop - 1);
+    }
+    issue_x(xp[0], mtop - 2);
+    issue_bred(std::integral_constant<int, 0>(), mtop, Z1());
+    auto step_dn = [&](int m, auto Zc) {
+      constexpr int ZM = decltype(Zc)::value;   // parity of m
+      lds_barrier();
+      red_plane(std::integral_constant<int, 0>(), m, Zc);   // b(m), x planes m-1..m+1
+      issue_bred(std::integral_constant<int, 0>(), m - 1, std::integral_constant<int, 1 - ZM>());
+      lds_barrier();
+      if (m + 1 <= k1 - 1) black_plane(bk[0], m + 1, std::integral_constant<int, 1 - ZM>());
+      if (m <= k1 - 1) issue_bblk(bk[0], m, Zc);
+      stage_x(xp[0], m - 2);
+      issue_x(xp[0], m - 3);
+    };
+    for (int m = mtop; m >= k0; m -= 2) {
+      step_dn(m, Z1());
+      step_dn(m - 1, Z0());
+    }
+    return 0.0;
+  }
   // prologue: planes k0-2 .. k0 staged; k0+1 .. k0+PF in flight (k0 is
   // odd).  The three staged planes' loads are all issued before the first is
   // staged: one memory round trip instead of three (small levels are bound
@@ -505,7 +543,7 @@ __device__ __forceinline__ double rb_fused_body(T* __restrict__ xo, const T* __r
 }
 
 // One block of the fused iteration; bid numbers the blocks of one grid.
-template <typename T, int TX, int TY, int MODE, int PF>
+template <typename T, int TX, int TY, int MODE, int PF, bool DOWN = false>
 __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, const T* __restrict__ xi,
                                                const T* __restrict__ b, const Lvl<T>& L, T omega,
                                                const FusedGrid& G, double* __restrict__ partial, const Lvl<T>& U,
@@ -523,7 +561,8 @@ __device__ __forceinline__ void rb_fused_block(int bid, T* __restrict__ xo, cons
   const int j0 = 1 + (rxy / G.gx) * TY;
   const int k0 = 1 + bz * G.kchunk;              // odd: kchunk is even
   const int k1 = min(k0 + G.kchunk, L.nz + 1);   // black planes k0 .. k1-1
-  const double acc = rb_fused_body<T, TX, TY, MODE, PF>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr, sbk, U, ext);
+  const double acc =
+      rb_fused_body<T, TX, TY, MODE, PF, DOWN>(xo, xi, b, L, omega, i0, j0, k0, k1, sx, sr, sbk, U, ext);
   if (MODE == 2 || MODE == 3) {
     __shared__ double sh[fused_threads(TX, TY) / 64];
     const double s = block_sum(acc, sh);
@@ -556,6 +595,19 @@ k_rb_fused(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b
            double* __restrict__ partial, Lvl<T> U, T* __restrict__ ext) {
   rb_fused_block<T, TX, TY, MODE, PF>(fused_xcd_bid(blockIdx.x, gridDim.x, G), xo, xi, b, L, omega, G, partial, U,
                                       ext);
+}
+
+// Mode 0 marching each chunk from the top down.  A launch that follows an
+// upward one on the same level first reads the planes that launch touched
+// last, which the memory-side cache still holds (a one-wave grid of blocks
+// marches all chunks at once): FP64 256^3 streams of the smoother's shape
+// 66 against 81 us per launch, FP32 512^3 300 against 312 us.
+template <typename T, int TX, int TY, int MODE, int PF>
+__global__ void __launch_bounds__(fused_threads(TX, TY))
+k_rb_fused_down(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b, Lvl<T> L, T omega,
+                FusedGrid G, double* __restrict__ partial, Lvl<T> U, T* __restrict__ ext) {
+  rb_fused_block<T, TX, TY, MODE, PF, true>(fused_xcd_bid(blockIdx.x, gridDim.x, G), xo, xi, b, L, omega, G, partial,
+                                            U, ext);
 }
 
 }  // namespace pepsmg

@@ -64,6 +64,7 @@ struct DevLevel {
   T *rw[3] = {nullptr, nullptr, nullptr}, *pp[3] = {nullptr, nullptr, nullptr}, *pn[3] = {nullptr, nullptr, nullptr};
   int rs[3] = {1, 1, 1}, ps[3] = {1, 1, 1};
   T *x = nullptr, *b = nullptr, *r = nullptr, *x2 = nullptr;
+  bool fused_up = false;                  // the last fused smoother launch on this level marched up
 
   Lvl<T> L() const {
     Lvl<T> l;

@@ -118,41 +118,59 @@ struct Ops {
   T* caller_sol = nullptr;
   Lvl<T> caller_lvl{};
 
-  template <int TX, int TY, int MODE, int PF>
+  // DOWN: mode 0 marching each chunk from the top (k_rb_fused_down), same
+  // decomposition and values
+  template <int TX, int TY, int MODE, int PF, bool DOWN = false>
   void fused_launch(DevLevel<T>& d, T omega, double* partial) {
     FusedGrid G = fused_grid(d, TX, TY, fused_resident<TX, TY, PF>());
     unsigned nb = (unsigned)(G.gx * G.gy * G.nch);
     const T* xi = MODE == 3 ? caller_x : d.x;
     const T* b = MODE == 3 ? caller_b : d.b;
     T* ext = MODE == 3 ? d.b : (MODE == 4 ? caller_sol : nullptr);
-    hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(fused_threads(TX, TY)), 0, s,
-                       fused_out ? fused_out : d.x2, xi, b, d.L(), omega, G, partial,
-                       (MODE == 3 || MODE == 4) ? caller_lvl : d.L(), ext);
+    if constexpr (DOWN) {
+      static_assert(MODE == 0 && PF == 1, "downward march: plain iterations, one plane of look-ahead");
+      hipLaunchKernelGGL((k_rb_fused_down<T, TX, TY, MODE, PF>), dim3(nb), dim3(fused_threads(TX, TY)), 0, s,
+                         fused_out ? fused_out : d.x2, xi, b, d.L(), omega, G, partial, d.L(), ext);
+    } else {
+      hipLaunchKernelGGL((k_rb_fused<T, TX, TY, MODE, PF>), dim3(nb), dim3(fused_threads(TX, TY)), 0, s,
+                         fused_out ? fused_out : d.x2, xi, b, d.L(), omega, G, partial,
+                         (MODE == 3 || MODE == 4) ? caller_lvl : d.L(), ext);
+    }
   }
 
   template <int TX, int TY, int PF>
-  void fused_mode(DevLevel<T>& d, T omega, int mode, double* partial) {
+  void fused_mode(DevLevel<T>& d, T omega, int mode, double* partial, bool down) {
     if (mode == 1) fused_launch<TX, TY, 1, PF>(d, omega, nullptr);
     else if constexpr (sizeof(T) == 8) {
       if (mode == 2) fused_launch<TX, TY, 2, PF>(d, omega, partial);
       else if (mode == 3) fused_launch<TX, TY, 3, PF>(d, omega, partial);
       else if (mode == 4) fused_launch<TX, TY, 4, PF>(d, omega, nullptr);
-      else fused_launch<TX, TY, 0, PF>(d, omega, nullptr);
+      else fused_plain<TX, TY, PF>(d, omega, down);
     } else {
-      fused_launch<TX, TY, 0, PF>(d, omega, nullptr);
+      fused_plain<TX, TY, PF>(d, omega, down);
     }
+  }
+  template <int TX, int TY, int PF>
+  void fused_plain(DevLevel<T>& d, T omega, bool down) {
+    if constexpr (PF == 1) {
+      if (down) {
+        fused_launch<TX, TY, 0, 1, true>(d, omega, nullptr);
+        return;
+      }
+    }
+    fused_launch<TX, TY, 0, PF>(d, omega, nullptr);
   }
 
   template <int TX>
-  void fused_tx(DevLevel<T>& d, T omega, int mode, int ty, double* partial) {
+  void fused_tx(DevLevel<T>& d, T omega, int mode, int ty, double* partial, bool down) {
     if (ty == 32) {
-      fused_mode<TX, 32, 1>(d, omega, mode, partial);
+      fused_mode<TX, 32, 1>(d, omega, mode, partial, down);
     } else if (ty == 16) {
-      if (prefetch == 2) fused_mode<TX, 16, 2>(d, omega, mode, partial);
-      else fused_mode<TX, 16, 1>(d, omega, mode, partial);
+      if (prefetch == 2) fused_mode<TX, 16, 2>(d, omega, mode, partial, false);
+      else fused_mode<TX, 16, 1>(d, omega, mode, partial, down);
     } else {
-      if (prefetch == 2) fused_mode<TX, 8, 2>(d, omega, mode, partial);
-      else fused_mode<TX, 8, 1>(d, omega, mode, partial);
+      if (prefetch == 2) fused_mode<TX, 8, 2>(d, omega, mode, partial, false);
+      else fused_mode<TX, 8, 1>(d, omega, mode, partial, down);
     }
   }
   template <int TX>
@@ -221,12 +239,22 @@ struct Ops {
     hipEventRecord(timer->ev[timer->used + 1], s);
     timer->used += 2;
   }
+  // A plain iteration that follows an upward launch on the same level
+  // marches down, so that it first reads the planes the previous launch
+  // touched last; every other fused launch marches up.
+  bool alternate = true;
+  bool next_down(DevLevel<T>& d, int mode) {
+    const bool down = alternate && mode == 0 && d.fused_up && (ty_used(d) == 32 || prefetch == 1);
+    d.fused_up = !down;
+    return down;
+  }
   void fused_iter(DevLevel<T>& d, T omega, int mode, double* partial = nullptr) {
     int ty = ty_used(d);
+    const bool down = next_down(d, mode);
     bool timed;
     timer_begin(d, mode, timed);
-    if (sizeof(T) == 4 && tx_used(d) == 64) fused_tx<64>(d, omega, mode, ty, partial);
-    else fused_tx<FTX>(d, omega, mode, ty, partial);
+    if (sizeof(T) == 4 && tx_used(d) == 64) fused_tx<64>(d, omega, mode, ty, partial, down);
+    else fused_tx<FTX>(d, omega, mode, ty, partial, down);
     timer_end(timed);
     std::swap(d.x, d.x2);
   }
@@ -234,10 +262,11 @@ struct Ops {
   void fused_iter_into(DevLevel<T>& d, T omega, T* out) {
     fused_out = out;
     int ty = ty_used(d);
+    const bool down = next_down(d, 0);
     bool timed;
     timer_begin(d, 0, timed);
-    if (sizeof(T) == 4 && tx_used(d) == 64) fused_tx<64>(d, omega, 0, ty, nullptr);
-    else fused_tx<FTX>(d, omega, 0, ty, nullptr);
+    if (sizeof(T) == 4 && tx_used(d) == 64) fused_tx<64>(d, omega, 0, ty, nullptr, down);
+    else fused_tx<FTX>(d, omega, 0, ty, nullptr, down);
     timer_end(timed);
     fused_out = nullptr;
   }

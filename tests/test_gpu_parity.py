@@ -68,6 +68,24 @@ def test_rbgs_iterator_bitwise(dims, lens, stretch, omega, nu):
     assert np.array_equal(xg, x)
 
 
+@pytest.mark.parametrize("nu", [2, 4])
+def test_rbgs_iterator_alternating_bitwise(nu):
+    """Consecutive iterations march their z-chunks upward, then downward
+    (k_rb_fused_down), on the 64 x 32 tiles of levels with 256 rows or more:
+    partial tiles in x and y, and an odd nz, so the top chunk ends on an
+    even plane and the downward march starts one plane above the domain.
+    Only x and y are stretched: the stretched-axis formula gives an odd
+    cell count a zero-width middle cell (infinite coefficients)."""
+    dims = (70, 260, 37)
+    g = O.Grid(*dims, a=(1.1, 1.05, 1.0))
+    A = g.matrix()
+    x, b = rand_padded(*dims), rand_padded(*dims)
+    xg = x.copy()
+    rbgs_iterator_poisson_matrix(xg, matrix.MatrixHeptadiagonal(0, A), b, sdm_from(g), nu, 1.6)
+    O.rbgs_iterator(x, A, b, nu, 1.6)
+    assert np.array_equal(xg, x)
+
+
 def test_rbgs_solver_matches():
     dims = (8, 8, 8)
     g = O.Grid(*dims)
