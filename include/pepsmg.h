@@ -45,7 +45,11 @@ enum {
 
 /* One subdomain's metrics (type subdomain, src/geometry.f90:59-118): cell
  * counts and the 1-D arrays dxm/dxg (cell size, centre spacing) of each
- * axis, each of length n+2 including the two ghost entries. */
+ * axis, each of length n+2 including the two ghost entries.  Every entry
+ * point that takes a grid returns PEPSMG_ERR_ARG unless dxm(1..n) and
+ * dxg(1..n+1) are positive and finite (the coefficients divide by them).
+ * The reference's stretched-axis formula sets only cells 1..n/2 and their
+ * mirrors (src/geometry.f90:190-197): a stretched axis needs an even n. */
 typedef struct pepsmg_grid {
   int32_t nx, ny, nz;
   const double *dxm, *dxg;

@@ -139,12 +139,36 @@ static void prolong_weights(const std::vector<double>& dmf, const std::vector<do
   s = 2;
 }
 
+// Grid descriptor: sizes, metric arrays, and every width dm(1..n) and centre
+// distance dg(1..n+1) positive and finite (the coefficients divide by them,
+// src/matrix.f90:68-80).  The reference's stretched-axis formula sets cells
+// 1..n/2 and their mirrors only (src/geometry.f90:190-197), so an odd cell
+// count leaves the middle width unset.
+int check_grid(const pepsmg_grid* g) {
+  if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1 || !g->dxm || !g->dxg || !g->dym || !g->dyg || !g->dzm || !g->dzg)
+    return fail(PEPSMG_ERR_ARG, "invalid grid descriptor");
+  const int n[3] = {g->nx, g->ny, g->nz};
+  const double* dm[3] = {g->dxm, g->dym, g->dzm};
+  const double* dg[3] = {g->dxg, g->dyg, g->dzg};
+  for (int a = 0; a < 3; a++) {
+    for (int i = 1; i <= n[a] + 1; i++) {
+      const bool ok_m = i > n[a] || (std::isfinite(dm[a][i]) && dm[a][i] > 0.0);
+      const bool ok_g = std::isfinite(dg[a][i]) && dg[a][i] > 0.0;
+      if (!ok_m || !ok_g)
+        return fail(PEPSMG_ERR_ARG,
+                    "grid metric %s(%d) of the %c axis is not positive and finite (a stretched axis needs an even "
+                    "cell count)",
+                    ok_m ? "dg" : "dm", i, "xyz"[a]);
+    }
+  }
+  return PEPSMG_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Hierarchy (multigrid_create and the 1-process level rules)
 int build_hierarchy(const pepsmg_grid* g, int nlevel, int aggr, int cga_level, std::vector<HostLevel>& lv,
                     int lvc[3]) {
-  if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1 || !g->dxm || !g->dxg || !g->dym || !g->dyg || !g->dzm || !g->dzg)
-    return fail(PEPSMG_ERR_ARG, "invalid grid descriptor");
+  if (int rc = check_grid(g)) return rc;
   if (nlevel <= 1)
     return fail(PEPSMG_ERR_LEVELS, "[Error] The number of level should be larger than 1. Current number: %d", nlevel);
   if (aggr < 0 || aggr > 2)

@@ -94,6 +94,7 @@ struct Hier {
   void release();
 };
 
+int check_grid(const pepsmg_grid* g);
 int build_hierarchy(const pepsmg_grid* g, int nlevel, int aggr, int cga_level, std::vector<HostLevel>& lv,
                     int lvc[3]);
 

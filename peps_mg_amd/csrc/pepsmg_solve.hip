@@ -1055,8 +1055,7 @@ struct OneLevel {
     if (s) hipStreamDestroy(s);
   }
   int init(const pepsmg_grid* g) {
-    if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1 || !g->dxm || !g->dxg || !g->dym || !g->dyg || !g->dzm || !g->dzg)
-      return fail(PEPSMG_ERR_ARG, "invalid grid descriptor");
+    if (int rc = check_grid(g)) return rc;
     h.n[0] = g->nx; h.n[1] = g->ny; h.n[2] = g->nz;
     const double* dms[3] = {g->dxm, g->dym, g->dzm};
     const double* dgs[3] = {g->dxg, g->dyg, g->dzg};

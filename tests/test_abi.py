@@ -72,3 +72,20 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+@pytest.mark.parametrize("ax,nx", [(1.1, 15), (1.0, 16)])
+def test_degenerate_grid_rejected_without_gpu(ax, nx):
+    """The grid check runs before any HIP call: a stretched axis with an odd
+    cell count (src/geometry.f90:190-197 leaves its middle width unset) and a
+    zero width written into a uniform axis are both PEPSMG_ERR_ARG."""
+    from peps_mg_amd import geometry
+    from peps_mg_amd.multigrid import Multigrid
+    sdm = geometry.geometry_subdomain_create(
+        geometry.geometry_domain_create(nx, 8, 8, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, ax=ax))
+    if ax == 1.0:
+        sdm.dxm[5] = 0.0
+    with pytest.raises(_lib.PepsmgError) as e:
+        Multigrid(sdm, 2, 10)
+    assert e.value.code == _lib.ERR_ARG
+    assert "x axis" in str(e.value)

@@ -74,8 +74,9 @@ def test_rbgs_iterator_alternating_bitwise(nu):
     (k_rb_fused_down), on the 64 x 32 tiles of levels with 256 rows or more:
     partial tiles in x and y, and an odd nz, so the top chunk ends on an
     even plane and the downward march starts one plane above the domain.
-    Only x and y are stretched: the stretched-axis formula gives an odd
-    cell count a zero-width middle cell (infinite coefficients)."""
+    Only x and y are stretched: the stretched-axis formula sets cells 1 ..
+    n/2 and their mirrors (src/geometry.f90:190-197), so an odd count leaves
+    the middle cell's width unset (0 in the oracle: infinite coefficients)."""
     dims = (70, 260, 37)
     g = O.Grid(*dims, a=(1.1, 1.05, 1.0))
     A = g.matrix()
@@ -291,6 +292,17 @@ def test_create_errors(nl, aggr, cga):
     with pytest.raises(_lib.PepsmgError) as e:
         Multigrid(sdm_from(g), nl, 10, aggr, cga)
     assert e.value.code == _lib.ERR_LEVELS
+
+
+def test_degenerate_grid_rejected_by_operators():
+    """Standalone operators check the metrics too: a stretched z axis with an
+    odd cell count has an unset (zero) middle width."""
+    g = O.Grid(8, 8, 9, a=(1.0, 1.0, 1.2))
+    assert g.dzm[5] == 0.0
+    x, b = rand_padded(8, 8, 9), rand_padded(8, 8, 9)
+    with pytest.raises(_lib.PepsmgError) as e:
+        rbgs_iterator_poisson_matrix(x, None, b, sdm_from(g), 1, 1.0)
+    assert e.value.code == _lib.ERR_ARG and "z axis" in str(e.value)
 
 
 def test_device_api_and_rhs_generator():
