@@ -141,6 +141,18 @@ void pepsmg_multigrid_destroy(pepsmg_ctx *ctx);
 int pepsmg_rbgs_iterator(const pepsmg_grid *grid, double *sol, const double *coeff,
                          const double *rhs, int32_t maxiteration, double omega);
 
+/* No counterpart in the reference (its iterator is FP64 only): the red/black
+ * iterator of pepsmg_rbgs_iterator in FP32 storage and arithmetic, i.e. the
+ * inner smoother of the mixed-precision path (precision 1), launch for launch
+ * as the solver runs it (the same tiles, alternating z-march directions).
+ * sol and rhs are the padded FP64 host arrays: they are rounded to FP32 on
+ * the way in and the FP32 result is widened into sol.  The coefficients are
+ * the FP64 metric rediscretisation rounded to FP32; there is no coeff check.
+ * For callers that want the FP32 smoother on its own, and to test it
+ * directly rather than through the outer refinement that corrects it. */
+int pepsmg_rbgs_iterator_fp32(const pepsmg_grid *grid, double *sol, const double *rhs,
+                              int32_t maxiteration, double omega);
+
 /* rbgs_solver_poisson_matrix (src/rbgs_poisson_matrix.f90:47-179): iterates
  * until sqrt(|r|^2/|r0|^2) <= tolerance or maxiteration iterations; *iters
  * receives the reference's loop variable on exit, *rel the last ratio. */

@@ -62,6 +62,7 @@ SIGNATURES = {
                                                        C.POINTER(Stats)]),
     "pepsmg_multigrid_destroy": (None, [C.c_void_p]),
     "pepsmg_rbgs_iterator": (C.c_int, [C.POINTER(Grid), _dp, _dp, _dp, C.c_int32, C.c_double]),
+    "pepsmg_rbgs_iterator_fp32": (C.c_int, [C.POINTER(Grid), _dp, _dp, C.c_int32, C.c_double]),
     "pepsmg_rbgs_solver": (C.c_int, [C.POINTER(Grid), _dp, _dp, _dp, C.c_int32, C.c_double, C.c_double,
                                      _ip, _dp]),
     "pepsmg_cg_iterator": (C.c_int, [C.POINTER(Grid), _dp, _dp, _dp, C.c_int32]),

@@ -1040,14 +1040,18 @@ int pepsmg_kernel_timing_read(const pepsmg_ctx* c, double* total_ms, int32_t* la
 // Standalone operators on one grid (host arrays).
 namespace pepsmg {
 
-struct OneLevel {
+// One grid on the device for the standalone operators: metrics, the level's
+// arrays in element type T (coefficients converted from the FP64 metrics),
+// and padded FP64 staging buffers for the caller's arrays.
+template <typename T>
+struct OneLevelT {
   HostLevel h;
-  DevLevel<double> d;
+  DevLevel<T> d;
   double *pad_a = nullptr, *pad_b = nullptr, *partial = nullptr, *sc = nullptr;
   int* flag = nullptr;
   hipStream_t s = nullptr;
   size_t np = 0;
-  ~OneLevel() {
+  ~OneLevelT() {
     hipFree(d.cw[0]); hipFree(d.cw[1]); hipFree(d.cw[2]);
     hipFree(d.ce[0]); hipFree(d.ce[1]); hipFree(d.ce[2]);
     hipFree(d.x); hipFree(d.b); hipFree(d.r); hipFree(d.x2);
@@ -1070,15 +1074,16 @@ struct OneLevel {
     d.pxy = d.px * d.ny;
     d.ncell = (size_t)d.pxy * d.nz;
     for (int a = 0; a < 3; a++) {
-      HIP_TRY(hipMalloc(&d.cw[a], h.cw[a].size() * sizeof(double)));
-      HIP_TRY(hipMalloc(&d.ce[a], h.ce[a].size() * sizeof(double)));
-      HIP_TRY(hipMemcpy(d.cw[a], h.cw[a].data(), h.cw[a].size() * sizeof(double), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(d.ce[a], h.ce[a].data(), h.ce[a].size() * sizeof(double), hipMemcpyHostToDevice));
+      const std::vector<T> cw(h.cw[a].begin(), h.cw[a].end()), ce(h.ce[a].begin(), h.ce[a].end());
+      HIP_TRY(hipMalloc(&d.cw[a], cw.size() * sizeof(T)));
+      HIP_TRY(hipMalloc(&d.ce[a], ce.size() * sizeof(T)));
+      HIP_TRY(hipMemcpy(d.cw[a], cw.data(), cw.size() * sizeof(T), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d.ce[a], ce.data(), ce.size() * sizeof(T), hipMemcpyHostToDevice));
     }
-    HIP_TRY(hipMalloc(&d.x, d.ncell * sizeof(double)));
-    HIP_TRY(hipMalloc(&d.b, d.ncell * sizeof(double)));
-    HIP_TRY(hipMalloc(&d.r, d.ncell * sizeof(double)));
-    HIP_TRY(hipMalloc(&d.x2, d.ncell * sizeof(double)));
+    HIP_TRY(hipMalloc(&d.x, d.ncell * sizeof(T)));
+    HIP_TRY(hipMalloc(&d.b, d.ncell * sizeof(T)));
+    HIP_TRY(hipMalloc(&d.r, d.ncell * sizeof(T)));
+    HIP_TRY(hipMalloc(&d.x2, d.ncell * sizeof(T)));
     np = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 2);
     HIP_TRY(hipMalloc(&pad_a, np * sizeof(double)));
     HIP_TRY(hipMalloc(&pad_b, np * sizeof(double)));
@@ -1088,13 +1093,13 @@ struct OneLevel {
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     return PEPSMG_OK;
   }
-  int upload(double* dst, const double* host) {
+  int upload(T* dst, const double* host) {
     HIP_TRY(hipMemcpy(pad_a, host, np * sizeof(double), hipMemcpyHostToDevice));
     pack(s, dst, pad_a, d, (int*)nullptr);
     HIP_TRY(hipStreamSynchronize(s));
     return PEPSMG_OK;
   }
-  int download(double* host, const double* src, int zero_ghosts) {
+  int download(double* host, const T* src, int zero_ghosts) {
     HIP_TRY(hipMemcpy(pad_b, host, np * sizeof(double), hipMemcpyHostToDevice));
     unpack(s, pad_b, src, d, zero_ghosts);
     HIP_TRY(hipStreamSynchronize(s));
@@ -1108,6 +1113,7 @@ struct OneLevel {
     return PEPSMG_OK;
   }
 };
+using OneLevel = OneLevelT<double>;
 
 // A device array of doubles freed when it goes out of scope, so every exit
 // path of an entry point releases it.
@@ -1166,6 +1172,19 @@ int pepsmg_rbgs_iterator(const pepsmg_grid* grid, double* sol, const double* coe
   if (rc) return rc;
   Ops<double> op{o.s};
   for (int it = 0; it < maxiteration; it++) op.fused_iter(o.d, omega, 0);
+  return o.download(sol, o.d.x, 0);
+}
+
+int pepsmg_rbgs_iterator_fp32(const pepsmg_grid* grid, double* sol, const double* rhs, int32_t maxiteration,
+                              double omega) {
+  OneLevelT<float> o;
+  int rc = o.init(grid);
+  if (!rc) rc = check_ghosts_host(sol, o.d.nx, o.d.ny, o.d.nz, "sol");
+  if (!rc) rc = o.upload(o.d.x, sol);
+  if (!rc) rc = o.upload(o.d.b, rhs);
+  if (rc) return rc;
+  Ops<float> op{o.s};
+  for (int it = 0; it < maxiteration; it++) op.fused_iter(o.d, (float)omega, 0);
   return o.download(sol, o.d.x, 0);
 }
 

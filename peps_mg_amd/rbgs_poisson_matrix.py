@@ -16,6 +16,17 @@ def rbgs_iterator_poisson_matrix(sol, a_poisson, rhs, dm, maxiteration, omega, i
                                                maxiteration, omega))
 
 
+def rbgs_iterator_fp32(sol, rhs, dm, maxiteration, omega):
+    """The red/black iterator in FP32 storage and arithmetic (the inner
+    smoother of the mixed-precision path; no counterpart in the reference):
+    sol and rhs are rounded to FP32, the FP32 result is written back into
+    sol (FP64 padded arrays, updated in place)."""
+    g = grid_descriptor(dm)
+    n = (dm.nx, dm.ny, dm.nz)
+    _lib.check(_lib.lib().pepsmg_rbgs_iterator_fp32(C.byref(g), _lib.padded_ptr(sol, *n, "sol"),
+                                                    _lib.padded_ptr(rhs, *n, "rhs"), maxiteration, omega))
+
+
 def rbgs_solver_poisson_matrix(sol, a_poisson, rhs, dm, maxiteration, tolerance, omega,
                                is_aggregated=(True,) * 3):
     """src/rbgs_poisson_matrix.f90:47-179; returns (iter, relative residual)."""
