@@ -150,14 +150,19 @@ struct Ops {
       fused_plain<TX, TY, PF>(d, omega, down);
     }
   }
+  // The decomposition is sized by the upward kernel's residency, so a tile
+  // shape whose downward kernel holds fewer blocks per CU (FP32 64 x 16: 137
+  // against 127 registers) marches up.
   template <int TX, int TY, int PF>
   void fused_plain(DevLevel<T>& d, T omega, bool down) {
     if constexpr (PF == 1) {
-      if (down) {
+      if (down && resident((const void*)k_rb_fused_down<T, TX, TY, 0, 1>, fused_threads(TX, TY)) ==
+                      fused_resident<TX, TY, 1>()) {
         fused_launch<TX, TY, 0, 1, true>(d, omega, nullptr);
         return;
       }
     }
+    d.fused_up = true;
     fused_launch<TX, TY, 0, PF>(d, omega, nullptr);
   }
 

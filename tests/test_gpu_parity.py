@@ -210,7 +210,7 @@ def workload_rhs(g, seed):
 @pytest.mark.parametrize("inner", [dict(), dict(inner_nu=1, inner_omega=0.0, inner_cycles=4),
                                    dict(inner_nu=2, inner_nu_post=1), dict(inner_nu=1, inner_nu_post=3, inner_cycles=3)],
                          ids=["default", "nu1-omega1", "v21", "v13"])
-@pytest.mark.parametrize("tx,ty", [(0, 0), (64, 0), (128, 32), (64, 32)])
+@pytest.mark.parametrize("tx,ty", [(0, 0), (64, 0), (128, 32), (64, 32), (64, 16)])
 @pytest.mark.parametrize("dims,lens", [((64, 64, 64), (1, 1, 1)), ((32, 32, 32), (1, 1, 2))])
 @pytest.mark.parametrize("rhs,tol", [("workload", 1e-10), ("random", 1e-12)])
 def test_mixed_precision_vs_fp64_oracle(dims, lens, tx, ty, inner, rhs, tol):
