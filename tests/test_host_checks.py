@@ -9,7 +9,8 @@ import pytest
 from peps_mg_amd import _lib, geometry, matrix
 from peps_mg_amd.cg_poisson_matrix import cg_iterator_poisson_matrix, cg_solver_poisson_matrix
 from peps_mg_amd.poisson_matrix_operator import mv_mul_poisson_matrix, vv_dot_3d_matrix
-from peps_mg_amd.rbgs_poisson_matrix import rbgs_iterator_poisson_matrix, rbgs_solver_poisson_matrix
+from peps_mg_amd.rbgs_poisson_matrix import (rbgs_iterator_fp32, rbgs_iterator_poisson_matrix,
+                                             rbgs_solver_poisson_matrix)
 
 NX, NY, NZ = 12, 10, 6
 
@@ -28,6 +29,7 @@ BAD = [np.zeros((NZ, NY, NX)),                          # interior only
 
 CALLS = {
     "rbgs_iterator": lambda s, a, b: rbgs_iterator_poisson_matrix(a, None, b, s, 1, 1.0),
+    "rbgs_iterator_fp32": lambda s, a, b: rbgs_iterator_fp32(a, b, s, 1, 1.0),
     "rbgs_solver": lambda s, a, b: rbgs_solver_poisson_matrix(a, None, b, s, 10, 1e-10, 1.0),
     "cg_iterator": lambda s, a, b: cg_iterator_poisson_matrix(a, None, b, s, 1),
     "cg_solver": lambda s, a, b: cg_solver_poisson_matrix(a, None, b, s, 10, 1e-10),
