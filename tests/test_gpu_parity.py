@@ -189,6 +189,36 @@ def test_vcycle_fp64_bitwise(dims, lens, stretch, nlevel, nu, omega, opts):
     assert np.array_equal(rg, ro)
 
 
+def test_stretched_tall_level_fp64_bitwise_and_mixed():
+    """Every axis stretched on a grid whose finest level has 512 rows: the
+    smoother's 64 x 32 tiles (partial in x on the next level) and the
+    refinement kernels' 32-row tiles, with non-uniform weights in every
+    kernel.  The lengths follow the cell counts and each stretch ratio keeps
+    the widest cell of an axis under twice the narrowest, so that the
+    point-smoothed V-cycle converges (in the unit cube, or with a ratio of
+    1.05 over 256 cells, the reference's cycle stalls).  FP64 equals the
+    oracle bit for bit; mixed precision lands within 1e-10 (max, relative)
+    of the oracle's FP64 solution, both solved to 1e-12 (the right-hand side
+    is random)."""
+    dims, lens, stretch, nl = (64, 512, 32), (0.125, 1.0, 0.0625), (1.03, 1.003, 1.04), 5
+    (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both(dims, lens, stretch, nl, 2, 1.0)
+    assert res.cycles == nc and res.converged
+    assert np.max(np.abs(res.history - hist)) <= 1e-12
+    assert np.array_equal(res.coarse_iters, cit)
+    assert np.array_equal(xg, xo) and np.array_equal(rg, ro)
+    g = O.Grid(*dims, l=lens, a=stretch)
+    A = g.matrix()
+    b = -O.uniform_field(*dims, 20210501)
+    xo, ro = O.padded(*dims), O.padded(*dims)
+    assert O.Multigrid(g, nl, 80).solve_vcycle(xo, ro, A, b, 2, 1e-12, 1.0)[0] < 80
+    mg = Multigrid(sdm_from(g), nl, 80, precision=1)
+    xm, rm = O.padded(*dims), O.padded(*dims)
+    res = mg.solve_vcycle(xm, rm, None, b, 2, 1e-12, 1.0)
+    assert res.converged and res.rho < 1e-12
+    err = np.abs(xm - xo).max() / np.abs(xo).max()
+    assert err <= 1e-10, err
+
+
 def test_vcycle_cg_smoother():
     (xo, ro, nc, hist, cit, res0), (xg, rg, res) = run_both((32, 32, 32), (1, 1, 1), (1, 1, 1), 3, 4, 1.0,
                                                             smoother=1)
