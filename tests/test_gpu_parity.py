@@ -37,7 +37,11 @@ def rand_padded(nx, ny, nz):
 
 
 CASES = [((16, 16, 16), (1, 1, 1), (1, 1, 1)), ((12, 10, 9), (1, 2, 1), (1, 1, 1)),
-         ((16, 8, 16), (1, 1, 1), (1.15, 1.0, 1.2)), ((5, 7, 3), (1, 1, 1), (1, 1, 1))]
+         ((16, 8, 16), (1, 1, 1), (1.15, 1.0, 1.2)), ((5, 7, 3), (1, 1, 1), (1, 1, 1)),
+         # one cell along an axis: a tile holds one cell, the pair (1, 2) reaches past nx, a
+         # z-chunk holds one plane
+         ((1, 1, 1), (1, 1, 1), (1, 1, 1)), ((1, 6, 3), (1, 1, 1), (1, 1, 1)),
+         ((7, 1, 2), (1, 1, 1), (1, 1, 1)), ((2, 3, 1), (1, 1, 1), (1, 1, 1))]
 
 
 @pytest.mark.parametrize("dims,lens,stretch", CASES)
@@ -89,10 +93,11 @@ def test_rbgs_iterator_alternating_bitwise(nu):
 
 
 FP32_CASES = [((70, 260, 37), (1.1, 1.05, 1.0)), ((128, 128, 33), (1.0, 1.0, 1.0)),
-              ((64, 64, 64), (1.15, 1.0, 1.2)), ((20, 12, 9), (1.0, 1.0, 1.0))]
+              ((64, 64, 64), (1.15, 1.0, 1.2)), ((20, 12, 9), (1.0, 1.0, 1.0)), ((1, 6, 3), (1.0, 1.0, 1.0))]
 
 
-@pytest.mark.parametrize("dims,stretch", FP32_CASES, ids=["128x32-partial", "128x32-whole", "64x8", "64x8-ragged"])
+@pytest.mark.parametrize("dims,stretch", FP32_CASES,
+                         ids=["128x32-partial", "128x32-whole", "64x8", "64x8-ragged", "one-column"])
 @pytest.mark.parametrize("omega,nu", [(1.0, 1), (1.35, 2), (1.6, 3)])
 def test_rbgs_iterator_fp32_against_oracle(dims, stretch, omega, nu):
     """The FP32 smoother of the mixed-precision path, launched as the solver
